@@ -1,0 +1,124 @@
+/*
+ * kdtree_hip.h -- C ABI of libkdtree_hip.so, the MI355X (gfx950) KD-tree build
+ * and exact k-NN query engine.
+ *
+ * The ABI is what the reference's binding layer would call in place of its Rust
+ * core: plain pointers and sizes, no framework types.  Each entry point names the
+ * reference interface it replaces (paths are into FujishigeTemma/kdtree-rs).
+ *
+ * Conventions
+ *   - Points and queries are row-major float64 matrices, n x ndim.
+ *   - Returned indices are int64 original row numbers; distances are float64.
+ *   - Missing neighbours are padded with (+inf, n_points), src/query.rs:425-428
+ *     and src/query.rs:488-491.
+ *   - Every function returns 0 on success or a KDT_ERR_* code; the message of
+ *     the last failure on the calling thread is kdt_last_error().  Codes 1-11
+ *     are the reference's KDTreeError variants (src/error.rs:3-16) and their
+ *     messages are the reference's Display strings (src/error.rs:18-51).
+ *   - A built tree is immutable; concurrent kdt_query calls on one tree from
+ *     several host threads are safe (the reference's frozen pyclass,
+ *     src/lib.rs:100 and src/lib.rs:185).  Host-pointer entry points run on the
+ *     calling thread's per-thread HIP stream.
+ */
+#ifndef KDTREE_HIP_H
+#define KDTREE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kdt_tree kdt_tree;
+
+enum kdt_error {
+  KDT_OK = 0,
+  KDT_ERR_EMPTY_DATA = 1,           /* KDTreeError::EmptyData */
+  KDT_ERR_INVALID_LEAFSIZE = 2,     /* KDTreeError::InvalidLeafsize */
+  KDT_ERR_INVALID_K = 3,            /* KDTreeError::InvalidK */
+  KDT_ERR_INVALID_METRIC = 4,       /* KDTreeError::InvalidMetric(p) */
+  KDT_ERR_INVALID_EPS = 5,          /* KDTreeError::InvalidEps(eps) */
+  KDT_ERR_INVALID_MAX_DISTANCE = 6, /* KDTreeError::InvalidMaxDistance(d) */
+  KDT_ERR_NON_FINITE = 7,           /* KDTreeError::NonFiniteData */
+  KDT_ERR_TOO_MANY_POINTS = 8,      /* KDTreeError::TooManyPoints(n) */
+  KDT_ERR_TOO_MANY_NODES = 9,       /* KDTreeError::TooManyNodes(n) */
+  KDT_ERR_INVALID_SHAPE = 10,       /* KDTreeError::InvalidShape(msg) */
+  KDT_ERR_DIMENSION_MISMATCH = 11,  /* KDTreeError::DimensionMismatch */
+  KDT_ERR_DEVICE = 100,             /* HIP runtime failure: no device, out of memory */
+  KDT_ERR_ARGUMENT = 101            /* null pointer or size overflow at the ABI */
+};
+
+/* Query flags. */
+#define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 leaf screen with a proven error margin,
+                                      FP64 recheck: identical results, fewer bytes */
+
+/* Node record exported for inspection (24 bytes, the reference's Node budget,
+ * src/tree.rs:97-100).  Leaf: a = first row, b = end row.  Inner: a = left child
+ * id, b = right child id. */
+typedef struct kdt_node_record {
+  double split_value;
+  uint32_t a;
+  uint32_t b;
+  uint32_t split_dim;
+  uint8_t is_leaf;
+  uint8_t order_by_box;
+  uint16_t pad;
+} kdt_node_record;
+
+/* Build a tree from host data, copied.  Replaces Tree::new / build::build,
+ * src/tree.rs:39-46 and src/build.rs:19-82 (validation order :25-56).
+ * `parallel` is accepted for parity with src/lib.rs:108; the device build is
+ * always parallel and its result does not depend on it (src/build.rs:345-359). */
+int kdt_build(const double* data, uint64_t n_points, uint64_t ndim, uint64_t leafsize,
+              int parallel, kdt_tree** out);
+
+/* Same, with `data` already in device memory (row-major, n_points x ndim).  The
+ * build runs on `stream` (a hipStream_t; 0 = the null stream).  The tree keeps
+ * its own copy. */
+int kdt_build_device(const double* d_data, uint64_t n_points, uint64_t ndim, uint64_t leafsize,
+                     void* stream, kdt_tree** out);
+
+/* Exact k-NN for host queries.  Replaces Tree::query, src/query.rs:29-78, with
+ * the dimension check of src/lib.rs:173-174 first.  max_distance = +inf means
+ * None.  out_dist and out_idx hold n_queries * k values, row-major. */
+int kdt_query(const kdt_tree* tree, const double* queries, uint64_t n_queries, uint64_t ndim,
+              uint64_t k, double p, double max_distance, double eps, uint32_t flags,
+              double* out_dist, int64_t* out_idx);
+
+/* Same, with queries and outputs in device memory; runs on `stream` and returns
+ * without synchronising unless validation needs the device (non-finite queries
+ * are reported by this call, which then synchronises `stream`). */
+int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_queries,
+                     uint64_t ndim, uint64_t k, double p, double max_distance, double eps,
+                     uint32_t flags, double* d_out_dist, int64_t* d_out_idx, void* stream);
+
+/* Rows in the caller's original order (n_points x ndim).  Replaces
+ * Tree::original_data, src/tree.rs:60-68 (the `data` getter, src/lib.rs:138-146). */
+int kdt_original_data(const kdt_tree* tree, double* out);
+
+/* Accessors, src/tree.rs:48-58 and src/lib.rs:123-136. */
+uint64_t kdt_ndim(const kdt_tree* tree);
+uint64_t kdt_n_points(const kdt_tree* tree);
+uint64_t kdt_leafsize(const kdt_tree* tree);
+uint64_t kdt_n_nodes(const kdt_tree* tree);
+
+/* Copy the tree's internal arrays to host memory (any pointer may be null):
+ * rows in tree order (n_points x ndim), original indices (n_points), nodes
+ * (n_nodes) and boxes (n_nodes x 2 x ndim, [lo | hi]).  These are the reference
+ * Tree fields data/indices/nodes/boxes (src/tree.rs:29-36); used by the build
+ * parity tests. */
+int kdt_export(const kdt_tree* tree, double* data, uint32_t* indices, kdt_node_record* nodes,
+               double* boxes);
+
+/* Release a tree and its device memory.  Replaces Drop for Tree. */
+void kdt_free(kdt_tree* tree);
+
+/* Message of the last error on this thread ("" if none).  Replaces
+ * <KDTreeError as Display>::fmt, src/error.rs:18-51. */
+const char* kdt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KDTREE_HIP_H */

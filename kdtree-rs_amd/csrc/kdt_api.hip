@@ -1,0 +1,328 @@
+// kdt_api.hip -- the C ABI (include/kdtree_hip.h): validation in the
+// reference's order, its error text, device memory and copies.
+//
+// Replaces the Rust core's public surface: Tree::new / build (src/tree.rs:39-46,
+// src/build.rs:19-82), Tree::query (src/query.rs:29-78), Tree::original_data
+// (src/tree.rs:60-68) and KDTreeError's Display (src/error.rs:18-51).
+
+#include <charconv>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kdtree_hip.h"
+#include "kdt_internal.h"
+
+struct kdt_tree {
+  kdt::Tree t;
+};
+
+namespace kdt {
+uint64_t count_tree_nodes(uint64_t n, uint64_t leafsize);
+}
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, std::string msg) {
+  g_last_error = std::move(msg);
+  return code;
+}
+
+// Rust's `{}` for f64: shortest round-trip digits, never an exponent, "inf",
+// "-inf", "NaN", "-0" for negative zero, no trailing ".0" on integral values.
+std::string rust_f64(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+  std::string out = std::signbit(x) ? "-" : "";
+  x = std::fabs(x);
+  if (x == 0.0) return out + "0";
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof buf, x, std::chars_format::scientific);
+  std::string sci(buf, r.ptr);
+  const size_t epos = sci.find('e');
+  std::string mant = sci.substr(0, epos);
+  const int exp10 = std::stoi(sci.substr(epos + 1));
+  std::string digits;
+  for (char c : mant)
+    if (c != '.') digits.push_back(c);
+  // value = 0.digits * 10^(exp10 + 1)
+  const int point = exp10 + 1;
+  if (point <= 0) {
+    out += "0." + std::string((size_t)(-point), '0') + digits;
+  } else if ((size_t)point >= digits.size()) {
+    out += digits + std::string((size_t)point - digits.size(), '0');
+  } else {
+    out += digits.substr(0, (size_t)point) + "." + digits.substr((size_t)point);
+  }
+  return out;
+}
+
+hipStream_t host_stream() { return hipStreamPerThread; }
+
+template <typename T>
+struct DevMem {
+  T* p = nullptr;
+  hipStream_t s = nullptr;
+  ~DevMem() {
+    if (p) (void)hipFreeAsync(p, s);
+  }
+  hipError_t alloc(size_t n, hipStream_t st) {
+    s = st;
+    return hipMallocAsync((void**)&p, std::max<size_t>(n, 1) * sizeof(T), st);
+  }
+};
+
+#define API_HIP(expr)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) return fail(KDT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Scalar part of the query validation (src/query.rs:38-57) after k and shape;
+// 0 when fine.  Fills the metric and derived parameters.
+int check_scalars(double p, double max_distance, double eps, kdt::QueryParams& qp, std::string& msg) {
+  if (!std::isfinite(eps) || eps < 0.0) {
+    msg = "eps must be finite and non-negative, got " + rust_f64(eps);
+    return KDT_ERR_INVALID_EPS;
+  }
+  if (std::isnan(max_distance) || max_distance < 0.0) {
+    msg = "max_distance must be finite and non-negative, or infinity, got " + rust_f64(max_distance);
+    return KDT_ERR_INVALID_MAX_DISTANCE;
+  }
+  // Metric::new, src/metric.rs:45-59
+  int metric;
+  if (std::isinf(p) && p > 0) metric = kdt::M_LINF;
+  else if (!std::isfinite(p) || p < 1.0) {
+    msg = "p must be finite and >= 1, or infinity, got " + rust_f64(p);
+    return KDT_ERR_INVALID_METRIC;
+  } else if (p == 1.0) metric = kdt::M_L1;
+  else if (p == 2.0) metric = kdt::M_L2;
+  else metric = kdt::M_LP;
+  auto reduce = [&](double x) {  // Metric::reduce, src/metric.rs:63-71
+    if (metric == kdt::M_L2) return x * x;
+    if (metric == kdt::M_LP) return std::pow(x, p);
+    return x;
+  };
+  qp.metric = metric;
+  qp.p = p;
+  qp.cutoff = reduce(max_distance);
+  qp.eps_factor = reduce(1.0 + eps);
+  return 0;
+}
+
+int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t nq, uint64_t ndim,
+               uint64_t k, double p, double max_distance, double eps, uint32_t flags, double* out_d,
+               int64_t* out_i, bool out_on_device, hipStream_t s) {
+  if (!tree) return fail(KDT_ERR_ARGUMENT, "null tree");
+  const kdt::Tree& t = tree->t;
+  if (ndim != t.d)  // src/lib.rs:74-84
+    return fail(KDT_ERR_DIMENSION_MISMATCH,
+                "dimension mismatch: expected " + std::to_string(t.d) + ", got " + std::to_string(ndim));
+  if (k == 0) return fail(KDT_ERR_INVALID_K, "k must be greater than zero");
+  if (nq == 0) return fail(KDT_ERR_INVALID_SHAPE, "queries must be a contiguous row-major matrix");
+  if (!q || !out_d || !out_i) return fail(KDT_ERR_ARGUMENT, "null buffer");
+  if (k > 0xFFFFFFFFull || nq > (~0ull) / k) return fail(KDT_ERR_ARGUMENT, "n_queries * k overflows");
+  kdt::QueryParams qp;
+  qp.k = k;
+  qp.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
+  std::string smsg;
+  const int scalar_rc = check_scalars(p, max_distance, eps, qp, smsg);
+
+  // Queries per launch: bounded so that device buffers stay modest for huge
+  // host batches.  Device-pointer calls run in one launch.
+  uint64_t tile = nq;
+  if (!q_on_device || !out_on_device) {
+    const uint64_t per = k * 16 + ndim * 8;
+    tile = std::max<uint64_t>(1, std::min<uint64_t>(nq, (1ull << 30) / per));
+  }
+  DevMem<int> flag;
+  API_HIP(flag.alloc(1, s));
+  API_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+  DevMem<double> dq, dd;
+  DevMem<int64_t> di;
+  if (!q_on_device) API_HIP(dq.alloc(tile * ndim, s));
+  if (!out_on_device) {
+    API_HIP(dd.alloc(tile * k, s));
+    API_HIP(di.alloc(tile * k, s));
+  }
+  for (uint64_t b = 0; b < nq; b += tile) {
+    const uint64_t m = std::min(tile, nq - b);
+    const double* qsrc = q + b * ndim;
+    const double* dqp = qsrc;
+    if (!q_on_device) {
+      API_HIP(hipMemcpyAsync(dq.p, qsrc, m * ndim * 8, hipMemcpyHostToDevice, s));
+      dqp = dq.p;
+    }
+    // all_finite, src/kernel.rs:67-74 (checked before eps, max_distance, p)
+    kdt::launch_check_finite(dqp, m * ndim, flag.p, s);
+    if (scalar_rc) {
+      int h = 0;
+      API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+      API_HIP(hipStreamSynchronize(s));
+      if (h) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
+      if (b + m >= nq) return fail(scalar_rc, smsg);
+      continue;
+    }
+    double* od = out_on_device ? out_d + b * k : dd.p;
+    int64_t* oi = out_on_device ? out_i + b * k : di.p;
+    kdt::QueryParams qpt = qp;
+    qpt.nq = m;
+    std::string msg;
+    const int rc = kdt::launch_query(t, qpt, dqp, od, oi, flag.p, s, msg);
+    if (rc) return fail(rc, msg);
+    if (!out_on_device) {
+      API_HIP(hipMemcpyAsync(out_d + b * k, dd.p, m * k * 8, hipMemcpyDeviceToHost, s));
+      API_HIP(hipMemcpyAsync(out_i + b * k, di.p, m * k * 8, hipMemcpyDeviceToHost, s));
+    }
+  }
+  int h = 0;
+  API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  API_HIP(hipStreamSynchronize(s));
+  if (h & 1) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
+  if (h & 2) return fail(KDT_ERR_DEVICE, "internal error: malformed tree met during the query");
+  return KDT_OK;
+}
+
+int build_impl(const double* data, bool on_device, uint64_t n, uint64_t d, uint64_t leafsize,
+               hipStream_t s, kdt_tree** out) {
+  if (!out) return fail(KDT_ERR_ARGUMENT, "null output");
+  *out = nullptr;
+  // Validation order of build::build, src/build.rs:25-43.
+  if (leafsize == 0) return fail(KDT_ERR_INVALID_LEAFSIZE, "leafsize must be greater than zero");
+  if (d == 0 || n == 0) return fail(KDT_ERR_EMPTY_DATA, "data must contain at least one point");
+  if (!data) return fail(KDT_ERR_ARGUMENT, "null data");
+  if (n > 0xFFFFFFFFull)
+    return fail(KDT_ERR_TOO_MANY_POINTS, "n_points must fit in a 32-bit index, got " + std::to_string(n));
+  const uint64_t n_nodes = kdt::count_tree_nodes(n, leafsize);
+  if (n_nodes > 0xFFFFFFFFull)
+    return fail(KDT_ERR_TOO_MANY_NODES, "tree would need " + std::to_string(n_nodes) +
+                                            " nodes, which exceeds the 32-bit node index; increase leafsize");
+  if (d > (~0ull) / n / 8) return fail(KDT_ERR_ARGUMENT, "n_points * ndim overflows");
+  kdt_tree* h = new kdt_tree();
+  h->t.n = n;
+  h->t.d = d;
+  h->t.leafsize = leafsize;
+  h->t.n_nodes = n_nodes;
+  (void)hipGetDevice(&h->t.device);
+  const double* src = data;
+  double* tmp = nullptr;
+  if (!on_device) {
+    hipError_t e = hipMalloc(&tmp, n * d * 8);
+    if (e == hipSuccess) e = hipMemcpyAsync(tmp, data, n * d * 8, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      if (tmp) (void)hipFree(tmp);
+      delete h;
+      return fail(KDT_ERR_DEVICE, std::string("device upload: ") + hipGetErrorString(e));
+    }
+    src = tmp;
+  }
+  std::string msg;
+  const int rc = kdt::build_tree(h->t, src, s, msg);
+  if (tmp) {
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(tmp);
+  }
+  if (rc) {
+    kdt_free(h);
+    return fail(rc, msg);
+  }
+  *out = h;
+  return KDT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kdt_build(const double* data, uint64_t n_points, uint64_t ndim, uint64_t leafsize, int parallel,
+              kdt_tree** out) {
+  (void)parallel;
+  return build_impl(data, false, n_points, ndim, leafsize, host_stream(), out);
+}
+
+int kdt_build_device(const double* d_data, uint64_t n_points, uint64_t ndim, uint64_t leafsize,
+                     void* stream, kdt_tree** out) {
+  return build_impl(d_data, true, n_points, ndim, leafsize, (hipStream_t)stream, out);
+}
+
+int kdt_query(const kdt_tree* tree, const double* queries, uint64_t n_queries, uint64_t ndim, uint64_t k,
+              double p, double max_distance, double eps, uint32_t flags, double* out_dist, int64_t* out_idx) {
+  return query_impl(tree, queries, false, n_queries, ndim, k, p, max_distance, eps, flags, out_dist, out_idx,
+                    false, host_stream());
+}
+
+int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_queries, uint64_t ndim,
+                     uint64_t k, double p, double max_distance, double eps, uint32_t flags,
+                     double* d_out_dist, int64_t* d_out_idx, void* stream) {
+  return query_impl(tree, d_queries, true, n_queries, ndim, k, p, max_distance, eps, flags, d_out_dist,
+                    d_out_idx, true, (hipStream_t)stream);
+}
+
+int kdt_original_data(const kdt_tree* tree, double* out) {
+  if (!tree || !out) return fail(KDT_ERR_ARGUMENT, "null argument");
+  const kdt::Tree& t = tree->t;
+  hipStream_t s = host_stream();
+  DevMem<double> d;
+  API_HIP(d.alloc(t.n * t.d, s));
+  kdt::launch_original_order(t, d.p, s);
+  API_HIP(hipMemcpyAsync(out, d.p, t.n * t.d * 8, hipMemcpyDeviceToHost, s));
+  API_HIP(hipStreamSynchronize(s));
+  return KDT_OK;
+}
+
+uint64_t kdt_ndim(const kdt_tree* tree) { return tree ? tree->t.d : 0; }
+uint64_t kdt_n_points(const kdt_tree* tree) { return tree ? tree->t.n : 0; }
+uint64_t kdt_leafsize(const kdt_tree* tree) { return tree ? tree->t.leafsize : 0; }
+uint64_t kdt_n_nodes(const kdt_tree* tree) { return tree ? tree->t.n_nodes : 0; }
+
+int kdt_export(const kdt_tree* tree, double* data, uint32_t* indices, kdt_node_record* nodes, double* boxes) {
+  if (!tree) return fail(KDT_ERR_ARGUMENT, "null tree");
+  const kdt::Tree& t = tree->t;
+  hipStream_t s = host_stream();
+  if (data) API_HIP(hipMemcpyAsync(data, t.rows, t.n * t.d * 8, hipMemcpyDeviceToHost, s));
+  if (indices) API_HIP(hipMemcpyAsync(indices, t.idx, t.n * 4, hipMemcpyDeviceToHost, s));
+  if (boxes) API_HIP(hipMemcpyAsync(boxes, t.boxes, t.n_nodes * 2 * t.d * 8, hipMemcpyDeviceToHost, s));
+  std::vector<kdt::DevNode> dn;
+  if (nodes) {
+    dn.resize(t.n_nodes);
+    API_HIP(hipMemcpyAsync(dn.data(), t.nodes, t.n_nodes * sizeof(kdt::DevNode), hipMemcpyDeviceToHost, s));
+  }
+  API_HIP(hipStreamSynchronize(s));
+  if (nodes) {
+    for (uint64_t id = 0; id < t.n_nodes; ++id) {
+      const kdt::DevNode& x = dn[id];
+      kdt_node_record r{};
+      if (x.b & kdt::LEAF_BIT) {
+        uint64_t end;
+        std::memcpy(&end, &x.split, 8);
+        r.is_leaf = 1;
+        r.a = x.a;
+        r.b = (uint32_t)end;
+      } else {
+        r.split_value = x.split;
+        r.a = (uint32_t)(id + 1);
+        r.b = x.a;
+        r.split_dim = x.b & kdt::DIM_MASK;
+        r.order_by_box = (x.b & kdt::OBB_BIT) ? 1 : 0;
+      }
+      nodes[id] = r;
+    }
+  }
+  return KDT_OK;
+}
+
+void kdt_free(kdt_tree* tree) {
+  if (!tree) return;
+  kdt::Tree& t = tree->t;
+  void* ptrs[] = {t.nodes, t.boxes, t.pts, t.pts32, t.idx, t.rows};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete tree;
+}
+
+const char* kdt_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

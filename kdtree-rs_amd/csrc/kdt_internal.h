@@ -1,0 +1,90 @@
+// kdt_internal.h -- layouts shared by the build, the query and the C ABI.
+//
+// Device tree image (all in HBM, owned by kdt_tree):
+//   nodes  DevNode[n_nodes]            16 B per node, preorder ids (left = id+1)
+//   boxes  f64[n_nodes][2][ndim]       tight box per node, [lo | hi]
+//   pts    f64 leaf buckets            leaf [start,end) of L rows stored
+//                                      coordinate-major: pts[start*ndim + c*L + o]
+//   pts32  f32 leaf buckets            same layout, for the FP32 screen
+//   idx    u32[n_points]               original row of tree position
+//   rows   f64[n_points][ndim]         rows in tree order (row-major), the
+//                                      reference's Tree::data
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace kdt {
+
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t OBB_BIT = 0x40000000u;
+constexpr uint32_t DIM_MASK = 0x3FFFFFFFu;
+
+// One node.  Inner: split = split value, a = right child id, b = split_dim
+// (| OBB_BIT when children are ordered by box distance).  Leaf: a = first row,
+// b = LEAF_BIT, split holds the end row as a 64-bit integer bit pattern.
+struct alignas(16) DevNode {
+  double split;
+  uint32_t a;
+  uint32_t b;
+};
+static_assert(sizeof(DevNode) == 16, "DevNode is one 16-byte load");
+
+enum MetricKind : int { M_L1 = 0, M_L2 = 1, M_LINF = 2, M_LP = 3 };
+
+struct Tree {
+  uint64_t n = 0, d = 0, leafsize = 0, n_nodes = 0;
+  uint32_t depth = 0;  // number of inner levels on the longest root-leaf path
+  int device = 0;
+  DevNode* nodes = nullptr;
+  double* boxes = nullptr;
+  double* pts = nullptr;
+  float* pts32 = nullptr;
+  uint32_t* idx = nullptr;
+  double* rows = nullptr;
+  double max_abs = 0.0;  // max |coordinate| over the data (root box)
+};
+
+// Build on the device from row-major f64 data already in device memory.
+// Returns 0 or a kdt_error code; `msg` receives the reason.
+int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg);
+
+struct QueryParams {
+  uint64_t nq = 0, k = 0;
+  int metric = M_L2;
+  double p = 2.0, cutoff = 0.0, eps_factor = 1.0;
+  bool fp32_screen = false;
+};
+
+// Launch the exact k-NN query; d_q [nq x d] f64, outputs [nq x k].
+// *d_err gets bit 2 if the kernel meets a malformed tree.
+int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
+                 int64_t* d_idx, int* d_err, hipStream_t s, std::string& msg);
+
+// Non-finite check of n doubles; sets *d_flag to 1 if any is NaN or infinite.
+void launch_check_finite(const double* d_x, uint64_t n, int* d_flag, hipStream_t s);
+
+// Scatter tree-order rows back to original order: out[idx[i]] = rows[i].
+void launch_original_order(const Tree& t, double* d_out, hipStream_t s);
+
+// Unpack the device node array into kdt_node_record-shaped host records.
+struct NodeRecord {
+  double split_value;
+  uint32_t a, b, split_dim;
+  uint8_t is_leaf, order_by_box;
+  uint16_t pad;
+};
+static_assert(sizeof(NodeRecord) == 24, "NodeRecord matches kdt_node_record");
+
+}  // namespace kdt
+
+#define KDT_HIP_CHECK(expr)                                                      \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      msg = std::string(#expr) + ": " + hipGetErrorString(e_);                   \
+      return 100;                                                                \
+    }                                                                            \
+  } while (0)

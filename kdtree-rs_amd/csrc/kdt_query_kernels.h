@@ -1,0 +1,563 @@
+// kdt_query_kernels.h -- exact k-NN traversal kernels for gfx950.
+//
+// Replaces Tree::query / run / Descent (src/query.rs:29-494) and the leaf
+// scans, point and box distances of src/kernel.rs:167-616.
+//
+// Work split: a group of G lanes (G in {16, 32, 64}, a power of two dividing
+// the 64-lane wavefront) owns one query at a time.  The group walks the tree
+// together -- every traversal value (node id, cell bound, k-th distance, stack
+// pointer) is the same in all G lanes, so a group never diverges internally --
+// and spreads each leaf over its lanes: lane o scores point o of the leaf from
+// the coordinate-major leaf bucket, so each coordinate is one coalesced load.
+//
+// The traversal is the reference's depth-first descent made iterative, visit
+// for visit: near child first (q[dim] - split <= 0 goes left, src/query.rs:264),
+// the far child gated by the split-plane bound and then the box bound, both
+// against one bound read after the near subtree (src/query.rs:278-286);
+// order_by_box nodes order and gate both children by box distance
+// (src/query.rs:296-327).  The per-axis cell bound (CellBound, :156-211) lives
+// in LDS; the recursion becomes an explicit stack with four entry kinds:
+//   FAR      deferred far child: plane gate, then box gate, then enter
+//   BOXFAR   deferred far child of an order_by_box node: box gate, enter
+//   BOXNEAR  deferred near child of an order_by_box node: box gate, descend
+//   RESTORE  undo of the cell-bound axis set when a far child was entered
+// Every inner node on the current path holds at most one deferred entry and
+// one RESTORE; a path has at most 32 inner nodes (n < 2^32), so 64 entries.
+//
+// Because visits, gates and bound updates happen in the reference's order,
+// the result is the reference's for every eps, not only eps = 0.  Distances
+// use the reference's per-width summation order (plan of src/kernel.rs:244-270):
+//   d <= 8  (Packed)   d4 full pairs: (a0+a2)+(a1+a3); d8 full chunks:
+//                      ((a0+a1)+(a2+a3))+((a4+a5)+(a6+a7)); all else in order
+//   9..16   (Folded)   pairwise hsum of the lane-folded 8-chunk, then the tail
+//   >= 17   (Streamed) running sum of per-8-chunk hsums, then the tail
+// where "full" means the point lies in a complete chunk of the leaf counted
+// from the leaf's first row (src/kernel.rs:447), P = 2 rows for d4, 8 for d8.
+//
+// The k best (distance, original index) pairs, lexicographic (src/query.rs:
+// 462-481), are either one entry per lane (k <= G, RegList) or a sorted array
+// in LDS (LdsList, any k that fits).
+#pragma once
+
+#include <type_traits>
+
+#include "kdt_internal.h"
+
+namespace kdt {
+
+constexpr int STK = 64;
+constexpr uint32_t KIND_FAR = 0, KIND_BOXFAR = 1, KIND_BOXNEAR = 2, KIND_RESTORE = 3;
+
+struct KArgs {
+  const DevNode* __restrict__ nodes;
+  const double* __restrict__ boxes;
+  const double* __restrict__ pts;
+  const float* __restrict__ pts32;
+  const uint32_t* __restrict__ idx;
+  const double* __restrict__ q;
+  double* __restrict__ out_d;
+  int64_t* __restrict__ out_i;
+  int* err;             // set to 2 on a malformed tree (never on a built one)
+  uint64_t nq;
+  uint32_t D, k, n_points, n_nodes;
+  double cutoff, eps_factor, p, inv_p;
+  double screen_e;      // FP32 screen: 4.0001 * sqrt(D) * 2^-24
+  double screen_gam;    // (1 + D u / (1 - D u)) (1 + 1e-14), u = 2^-24
+  double screen_shrink; // 1 + 2 (D + 3) 2^-53
+  double max_abs;       // max |coordinate| of the tree
+};
+
+// ------------------------------------------------------------ metric algebra
+template <int M>
+__device__ __forceinline__ double reduce_d(double x, double p) {  // Metric::reduce, src/metric.rs:63-71
+  if constexpr (M == M_L2) return x * x;
+  else if constexpr (M == M_LP) return pow(x, p);
+  else return x;
+}
+template <int M>
+__device__ __forceinline__ double restore_d(double x, double inv_p) {  // Metric::restore, :73-80
+  if constexpr (M == M_L2) return sqrt(x);
+  else if constexpr (M == M_LP) return pow(x, inv_p);
+  else return x;
+}
+template <int M>
+__device__ __forceinline__ double fold_d(double dist, double axis) {  // Metric::fold, :82-88
+  if constexpr (M == M_LINF) return fmax(dist, axis);
+  else return dist + axis;
+}
+template <int M>
+__device__ __forceinline__ double replace_axis_d(double dist, double old, double nw) {  // :90-98
+  if constexpr (M == M_LINF) return fmax(dist, nw);
+  else return dist - old + nw;
+}
+// One axis of a leaf distance: |delta|, delta^2 or |delta|^p (axes_lanes,
+// src/kernel.rs:22-29; point_dist_lp :207-218).
+template <int M>
+__device__ __forceinline__ double axis_term(double delta, double p) {
+  if constexpr (M == M_L2) return delta * delta;
+  else if constexpr (M == M_LP) return pow(fabs(delta), p);
+  else return fabs(delta);
+}
+__device__ __forceinline__ double vmax_d(double a, double b) { return b > a ? b : a; }  // src/simd.rs:12-15
+__device__ __forceinline__ double axis_offset_d(double q, double lo, double hi) {      // src/layout.rs:145-149
+  return fmax(fmax(lo - q, q - hi), 0.0);
+}
+__device__ __forceinline__ double hsum8(const double v[8]) {  // src/simd.rs:25-30
+  return ((v[0] + v[4]) + (v[2] + v[6])) + ((v[1] + v[5]) + (v[3] + v[7]));
+}
+
+// box_dist, src/kernel.rs:565-616.  `lo` points at the node's [lo | hi].
+template <int M, int DT>
+__device__ __forceinline__ double box_dist_d(const double* q, const double* __restrict__ lo,
+                                             uint32_t Drt, double p) {
+  const uint32_t D = DT > 0 ? DT : Drt;
+  const double* hi = lo + D;
+  if constexpr (M == M_L1 || M == M_L2) {
+    const uint32_t nch = D / 8;
+    double dist = 0.0;
+    if (nch) {
+      double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      for (uint32_t ch = 0; ch < nch; ++ch) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t i = ch * 8 + j;
+          const double off = vmax_d(vmax_d(lo[i] - q[i], q[i] - hi[i]), 0.0);
+          acc[j] += M == M_L2 ? off * off : fabs(off);
+        }
+      }
+      dist = hsum8(acc);
+    }
+#pragma unroll
+    for (uint32_t i = nch * 8; i < D; ++i) dist += reduce_d<M>(axis_offset_d(q[i], lo[i], hi[i]), p);
+    return dist;
+  } else if constexpr (M == M_LINF) {
+    double worst = 0.0;
+#pragma unroll
+    for (uint32_t i = 0; i < D; ++i) {
+      const double off = axis_offset_d(q[i], lo[i], hi[i]);
+      if (off > worst) worst = off;
+    }
+    return worst;
+  } else {
+    double dist = 0.0;
+    for (uint32_t i = 0; i < D; ++i) {
+      const double off = axis_offset_d(q[i], lo[i], hi[i]);
+      if (off > 0.0) dist += pow(off, p);
+    }
+    return dist;
+  }
+}
+
+// Reduced distance from q to point o of a leaf bucket of L rows
+// (coordinate c at base[c*L + o]), in the reference's order for width D.
+template <int M, int DT>
+__device__ __forceinline__ double point_dist_d(const double* q, const double* __restrict__ base,
+                                               uint32_t L, uint32_t o, uint32_t Drt, bool full,
+                                               double p) {
+  const uint32_t D = DT > 0 ? DT : Drt;
+  auto term = [&](uint32_t c) -> double { return axis_term<M>(base[(size_t)c * L + o] - q[c], p); };
+  if constexpr (M == M_LINF) {
+    double m = 0.0;
+#pragma unroll
+    for (uint32_t c = 0; c < D; ++c) {
+      const double t = term(c);
+      if (t > m) m = t;
+    }
+    return m;
+  } else if constexpr (M == M_LP) {
+    double s = 0.0;
+    for (uint32_t c = 0; c < D; ++c) s += term(c);
+    return s;
+  } else {
+    if (D <= 8) {
+      if (D == 4 && full) {
+        const double a0 = term(0), a1 = term(1), a2 = term(2), a3 = term(3);
+        return (a0 + a2) + (a1 + a3);
+      }
+      if (D == 8 && full) {
+        double a[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) a[c] = term(c);
+        return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+      }
+      double s = term(0);
+#pragma unroll
+      for (uint32_t c = 1; c < D; ++c) s += term(c);
+      return s;
+    }
+    if (D <= 16) {
+      double w[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) w[j] = term(j);
+      if (D == 16) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) w[j] = w[j] + term(8 + j);
+      }
+      double s = hsum8(w);
+#pragma unroll
+      for (uint32_t c = 8; c < (D == 16 ? 8u : D); ++c) s += term(c);
+      return s;
+    }
+    double s = 0.0;
+    uint32_t c = 0;
+    for (; c + 8 <= D; c += 8) {
+      double w[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) w[j] = term(c + j);
+      s += hsum8(w);
+    }
+    for (; c < D; ++c) s += term(c);
+    return s;
+  }
+}
+
+// FP32 squared distance for the screen (any order; its error is bounded).
+template <int DT>
+__device__ __forceinline__ float point_dist32(const float* q32, const float* __restrict__ base,
+                                              uint32_t L, uint32_t o) {
+  float s = 0.0f;
+#pragma unroll
+  for (int c = 0; c < DT; ++c) {
+    const float d = base[(size_t)c * L + o] - q32[c];
+    s += d * d;
+  }
+  return s;
+}
+
+// Smallest FP32 value T such that an FP32 squared distance above T proves the
+// reference's FP64 reduced distance is above `bound`.  With t_i the FP32
+// difference, |t_i - delta_i| <= u(2+u)(|p_i|+|q_i|), so ||t - delta|| <= E =
+// 4.0001 sqrt(D) u M; the FP32 sum S satisfies S <= (1+g)||t||^2 + D 2^-149;
+// the FP64 sum is >= (1-(D+2)2^-53)||delta||^2.  Hence S > (1+g)(sqrt(bound')
+// + E)^2 + D 2^-149 implies the FP64 distance exceeds bound.
+__device__ __forceinline__ float screen_threshold(double bound, double E, const KArgs& a) {
+  if (!(bound < INFINITY)) return INFINITY;
+  const double b = bound * a.screen_shrink + (double)a.D * 1e-300;
+  double t = sqrt(b) + E;
+  t = t * t * a.screen_gam + (double)a.D * 1.5e-45;
+  return __double2float_ru(t);
+}
+
+// ------------------------------------------------------------------ k lists
+// One entry per lane (k <= G).  Lane j holds the j-th best.
+template <int G>
+struct RegList {
+  double kd;
+  uint32_t ki;
+  uint32_t cnt;
+  __device__ __forceinline__ void reset() { kd = INFINITY; ki = 0xFFFFFFFFu; cnt = 0; }
+  // BestK::consider, src/query.rs:462-481.  Returns the new bound.
+  __device__ __forceinline__ double insert(double d, uint32_t x, double bound, uint32_t lane,
+                                           uint32_t gshift, uint64_t gmask, const KArgs& a) {
+    const bool less = lane < cnt && (kd < d || (kd == d && ki < x));
+    const uint32_t pos = (uint32_t)__popcll((__ballot(less) >> gshift) & gmask);
+    if (pos < a.k) {
+      const double ud = __shfl_up(kd, 1, G);
+      const uint32_t ui = __shfl_up(ki, 1, G);
+      if (lane == pos) { kd = d; ki = x; }
+      else if (lane > pos && lane < a.k) { kd = ud; ki = ui; }
+      if (cnt < a.k) ++cnt;
+      if (cnt == a.k) bound = fmin(__shfl(kd, a.k - 1, G), a.cutoff);  // BestK::bound, :453-459
+    }
+    return bound;
+  }
+  // write_results, src/query.rs:483-493
+  template <int M>
+  __device__ __forceinline__ void write(uint64_t qi, uint32_t lane, const KArgs& a) const {
+    if (lane < a.k) {
+      const bool have = lane < cnt;
+      a.out_d[qi * a.k + lane] = have ? restore_d<M>(kd, a.inv_p) : INFINITY;
+      a.out_i[qi * a.k + lane] = have ? (int64_t)ki : (int64_t)a.n_points;
+    }
+  }
+};
+
+// Sorted array in LDS for any k that fits (one group of 64 lanes per block).
+struct LdsList {
+  double* ld;
+  uint32_t* li;
+  uint32_t cnt;
+  __device__ __forceinline__ void reset() { cnt = 0; }
+  __device__ __forceinline__ double insert(double d, uint32_t x, double bound, uint32_t lane,
+                                           uint32_t, uint64_t, const KArgs& a) {
+    uint32_t pos = 0;
+    for (uint32_t j0 = 0; j0 < cnt; j0 += 64) {
+      const uint32_t j = j0 + lane;
+      bool less = false;
+      if (j < cnt) less = ld[j] < d || (ld[j] == d && li[j] < x);
+      pos += (uint32_t)__popcll(__ballot(less));
+    }
+    if (pos >= a.k) return bound;
+    const uint32_t ncnt = cnt < a.k ? cnt + 1 : a.k;
+    // Shift [pos, ncnt-1) up one slot, highest chunk first: each chunk reads
+    // below what it writes and above what the next chunk writes.
+    for (int64_t jt = (int64_t)ncnt - 1; jt > (int64_t)pos; jt -= 64) {
+      const int64_t j = jt - (int64_t)lane;
+      double vd = 0.0;
+      uint32_t vi = 0;
+      const bool mv = j > (int64_t)pos;
+      if (mv) { vd = ld[j - 1]; vi = li[j - 1]; }
+      __asm__ volatile("" ::: "memory");
+      if (mv) { ld[j] = vd; li[j] = vi; }
+      __asm__ volatile("" ::: "memory");
+    }
+    ld[pos] = d;
+    li[pos] = x;
+    __asm__ volatile("" ::: "memory");
+    cnt = ncnt;
+    if (cnt == a.k) bound = fmin(ld[a.k - 1], a.cutoff);
+    return bound;
+  }
+  template <int M>
+  __device__ __forceinline__ void write(uint64_t qi, uint32_t lane, const KArgs& a) const {
+    for (uint32_t j = lane; j < a.k; j += 64) {
+      const bool have = j < cnt;
+      a.out_d[qi * a.k + j] = have ? restore_d<M>(ld[j], a.inv_p) : INFINITY;
+      a.out_i[qi * a.k + j] = have ? (int64_t)li[j] : (int64_t)a.n_points;
+    }
+  }
+};
+
+// ------------------------------------------------------------------ kernel
+// DT: compile-time ndim (0 = runtime a.D).  G: lanes per query.  M: metric.
+// SCREEN: FP32 leaf screen + FP64 recheck (L2, DT > 0 only).  LDSLIST: k list
+// in LDS (G must be 64, one group per block; dynamic LDS holds the list and,
+// for DT == 0, the cell-bound axes).
+template <int DT, int G, int M, bool SCREEN, bool LDSLIST>
+__device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s_q, double* s_sa,
+                                         double* s_sb, uint32_t* s_sn, uint32_t* s_sm,
+                                         double* l_d, uint32_t* l_i) {
+  constexpr int GPB = 256 / G;
+  constexpr int DL = DT > 0 ? DT : 1;
+  const uint32_t lane = threadIdx.x & (G - 1);
+  const uint32_t grp = threadIdx.x / G;
+  const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const uint32_t D = DT > 0 ? DT : a.D;
+  const uint32_t gpb = LDSLIST ? 1u : (uint32_t)GPB;
+  const uint64_t gid = (uint64_t)blockIdx.x * gpb + grp;
+  const uint64_t ngroups = (uint64_t)gridDim.x * gpb;
+  const double eps = a.eps_factor;
+
+  typename std::conditional<LDSLIST, LdsList, RegList<G>>::type best;
+  if constexpr (LDSLIST) { best.ld = l_d; best.li = l_i; }
+
+  for (uint64_t qi = gid; qi < a.nq; qi += ngroups) {
+    const double* qg = a.q + qi * D;
+    double qr[DL];
+    float q32[DL];
+    const double* q;
+    double qmax = 0.0;
+    if constexpr (DT > 0) {
+#pragma unroll
+      for (int c = 0; c < DT; ++c) {
+        qr[c] = qg[c];
+        s_q[c] = qr[c];
+        axes[c] = 0.0;
+        q32[c] = __double2float_rn(qr[c]);
+        qmax = fmax(qmax, fabs(qr[c]));
+      }
+      q = qr;
+    } else {
+      q = qg;
+      for (uint32_t c = 0; c < D; ++c) axes[c] = 0.0;
+    }
+    (void)q32;
+    best.reset();
+    double bound = a.cutoff;  // BestK::bound with an empty list
+    float t32 = INFINITY;
+    double screen_e = 0.0;
+    bool screen_on = false;
+    if constexpr (SCREEN) {
+      screen_on = qmax <= 1e18 && a.max_abs <= 1e18;
+      screen_e = a.screen_e * fmax(qmax, a.max_abs) + 1e-43;
+      t32 = screen_threshold(bound, screen_e, a);
+    }
+
+    // CellBound::start / seed, src/query.rs:170-187
+    double cell = 0.0;
+    if (box_dist_d<M, DT>(q, a.boxes, D, a.p) != 0.0) {
+      for (uint32_t c = 0; c < D; ++c) {
+        const double ax = reduce_d<M>(axis_offset_d(q[c], a.boxes[c], a.boxes[D + c]), a.p);
+        axes[c] = ax;
+        cell = fold_d<M>(cell, ax);
+      }
+    }
+    int sp = 0;
+    uint32_t node = 0;
+    bool active = cell * eps <= bound;  // Descent::run, src/query.rs:227-229
+    while (active) {
+      const DevNode nd = a.nodes[node];
+      const bool leaf = (nd.b & LEAF_BIT) != 0;
+      // Preorder ids grow downwards and leaves lie inside the point range;
+      // anything else ends the query instead of walking off the arrays.
+      if (leaf ? ((uint64_t)__double_as_longlong(nd.split) > a.n_points ||
+                  (uint64_t)__double_as_longlong(nd.split) < nd.a)
+               : (nd.a <= node || nd.a >= a.n_nodes || (nd.b & DIM_MASK) >= D)) {
+        if (lane == 0) atomicOr(a.err, 2);
+        break;
+      }
+      if (!leaf) {
+        // descend, src/query.rs:255-287
+        const uint32_t dim = nd.b & DIM_MASK;
+        const double qd = DT > 0 ? s_q[dim] : qg[dim];
+        const double diff = qd - nd.split;
+        const bool goleft = diff <= 0.0;
+        const uint32_t near = goleft ? node + 1 : nd.a;
+        const uint32_t far = goleft ? nd.a : node + 1;
+        const double far_axis = reduce_d<M>(fabs(diff), a.p);
+        const double old = axes[dim];
+        const double far_dist = replace_axis_d<M>(cell, old, far_axis);
+        if (nd.b & OBB_BIT) {
+          // descend_by_box, src/query.rs:298-327
+          const double nb = box_dist_d<M, DT>(q, a.boxes + (size_t)near * 2 * D, D, a.p);
+          const double fb = box_dist_d<M, DT>(q, a.boxes + (size_t)far * 2 * D, D, a.p);
+          if (nb <= fb) {
+            s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_BOXFAR | (dim << 2);
+            ++sp;
+            if (nb * eps <= bound) { node = near; continue; }
+          } else {
+            s_sa[sp] = cell; s_sb[sp] = 0.0; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;
+            ++sp;
+            if (fb * eps <= bound) {  // enter_far, src/query.rs:329-341
+              s_sa[sp] = old; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
+              ++sp;
+              axes[dim] = far_axis;
+              node = far;
+              cell = far_dist;
+              continue;
+            }
+          }
+        } else {
+          s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_FAR | (dim << 2);
+          ++sp;
+          node = near;
+          continue;
+        }
+      } else {
+        // scan_leaf, src/query.rs:343-353; leaf scans, src/kernel.rs:359-563
+        const uint32_t start = nd.a;
+        const uint32_t L = (uint32_t)__double_as_longlong(nd.split) - start;
+        const double* base = a.pts + (size_t)start * D;
+        const uint32_t full = D == 4 ? (L & ~1u) : D == 8 ? (L & ~7u) : L;
+        for (uint32_t c0 = 0; c0 < L; c0 += G) {
+          const uint32_t o = c0 + lane;
+          double dist = INFINITY;
+          bool hit = false;
+          if (o < L) {
+            bool cand = true;
+            if constexpr (SCREEN) {
+              if (screen_on)
+                cand = point_dist32<DL>(q32, a.pts32 + (size_t)start * D, L, o) <= t32;
+            }
+            if (cand) {
+              dist = point_dist_d<M, DT>(q, base, L, o, D, o < full, a.p);
+              hit = dist <= bound;
+            }
+          }
+          uint64_t m = (__ballot(hit) >> gshift) & gmask;
+          if (m) {
+            const uint32_t myidx = hit ? a.idx[start + o] : 0u;
+            do {
+              const int f = __ffsll((unsigned long long)m) - 1;
+              const double df = __shfl(dist, f, G);
+              const uint32_t xf = __shfl(myidx, f, G);
+              bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
+              m &= m - 1;
+              m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
+            } while (m);
+            if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
+          }
+        }
+      }
+      // Unwind to the next deferred visit.
+      active = false;
+      while (sp > 0) {
+        --sp;
+        const uint32_t meta = s_sm[sp], en = s_sn[sp];
+        const double ea = s_sa[sp], eb = s_sb[sp];
+        const uint32_t kind = meta & 3u, edim = meta >> 2;
+        if (kind == KIND_RESTORE) { axes[edim] = ea; continue; }
+        if (kind == KIND_BOXNEAR) {
+          if (box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound) {
+            node = en; cell = ea; active = true; break;
+          }
+          continue;
+        }
+        bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
+        if (ok) ok = box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound;
+        if (ok) {
+          s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
+          ++sp;
+          axes[edim] = ea;
+          node = en;
+          cell = eb;
+          active = true;
+          break;
+        }
+      }
+    }
+    best.template write<M>(qi, lane, a);
+  }
+}
+
+template <int DT, int G, int M, bool SCREEN>
+__global__ __launch_bounds__(256) void knn_reg_kernel(KArgs a) {
+  constexpr int GPB = 256 / G;
+  constexpr int DL = DT > 0 ? DT : 1;
+  __shared__ double s_sa[GPB][STK];
+  __shared__ double s_sb[GPB][STK];
+  __shared__ uint32_t s_sn[GPB][STK];
+  __shared__ uint32_t s_sm[GPB][STK];
+  __shared__ double s_ax[GPB][DL];
+  __shared__ double s_q[GPB][DL];
+  const uint32_t grp = threadIdx.x / G;
+  knn_body<DT, G, M, SCREEN, false>(a, s_ax[grp], s_q[grp], s_sa[grp], s_sb[grp], s_sn[grp], s_sm[grp],
+                                    nullptr, nullptr);
+}
+
+// Runtime ndim, k list in LDS; one 64-lane group per block.  Dynamic LDS:
+// [axes D x f64][list k x f64][list k x u32].
+template <int M>
+__global__ __launch_bounds__(64) void knn_lds_kernel(KArgs a) {
+  extern __shared__ double dyn[];
+  __shared__ double s_sa[STK];
+  __shared__ double s_sb[STK];
+  __shared__ uint32_t s_sn[STK];
+  __shared__ uint32_t s_sm[STK];
+  double* axes = dyn;
+  double* l_d = dyn + a.D;
+  uint32_t* l_i = reinterpret_cast<uint32_t*>(l_d + a.k);
+  knn_body<0, 64, M, false, true>(a, axes, nullptr, s_sa, s_sb, s_sn, s_sm, l_d, l_i);
+}
+
+// Dispatch entry points, one per metric translation unit.
+int launch_knn_l1(const KArgs& a, int G, hipStream_t s);
+int launch_knn_l2(const KArgs& a, int G, bool screen, hipStream_t s);
+int launch_knn_linf(const KArgs& a, int G, hipStream_t s);
+int launch_knn_lds(const KArgs& a, int metric, hipStream_t s);
+
+template <int DT, int G, int M, bool SCREEN>
+inline void launch_reg(const KArgs& a, hipStream_t s) {
+  constexpr int GPB = 256 / G;
+  uint64_t blocks = (a.nq + GPB - 1) / GPB;
+  if (blocks > (1u << 20)) blocks = 1u << 20;
+  knn_reg_kernel<DT, G, M, SCREEN><<<(uint32_t)blocks, 256, 0, s>>>(a);
+}
+
+// Widths with a compiled kernel: the reference's own monomorphised list
+// (src/build.rs:60); other widths run the runtime-ndim kernel.
+template <int G, int M, bool SCREEN>
+inline int launch_reg_width(const KArgs& a, hipStream_t s) {
+  switch (a.D) {
+    case 1: launch_reg<1, G, M, SCREEN>(a, s); return 0;
+    case 2: launch_reg<2, G, M, SCREEN>(a, s); return 0;
+    case 3: launch_reg<3, G, M, SCREEN>(a, s); return 0;
+    case 4: launch_reg<4, G, M, SCREEN>(a, s); return 0;
+    case 8: launch_reg<8, G, M, SCREEN>(a, s); return 0;
+    case 16: launch_reg<16, G, M, SCREEN>(a, s); return 0;
+    default: return -1;
+  }
+}
+
+}  // namespace kdt

@@ -1,0 +1,15 @@
+// kdt_query_l1.hip -- Manhattan (p = 1) traversal kernels.
+#include "kdt_query_kernels.h"
+
+namespace kdt {
+
+int launch_knn_l1(const KArgs& a, int G, hipStream_t s) {
+  switch (G) {
+    case 16: return launch_reg_width<16, M_L1, false>(a, s);
+    case 32: return launch_reg_width<32, M_L1, false>(a, s);
+    case 64: return launch_reg_width<64, M_L1, false>(a, s);
+  }
+  return -1;
+}
+
+}  // namespace kdt

@@ -1,0 +1,19 @@
+// kdt_query_lds.hip -- runtime-width kernels with the k list in LDS: every
+// metric (general p included), any ndim, any k whose list fits in LDS.
+#include "kdt_query_kernels.h"
+
+namespace kdt {
+
+int launch_knn_lds(const KArgs& a, int metric, hipStream_t s) {
+  const size_t shm = (size_t)a.D * 8 + (size_t)a.k * 12;
+  uint64_t blocks = a.nq < (1u << 20) ? a.nq : (1u << 20);
+  switch (metric) {
+    case M_L1: knn_lds_kernel<M_L1><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
+    case M_L2: knn_lds_kernel<M_L2><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
+    case M_LINF: knn_lds_kernel<M_LINF><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
+    case M_LP: knn_lds_kernel<M_LP><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
+  }
+  return -1;
+}
+
+}  // namespace kdt

@@ -1,0 +1,172 @@
+"""GPU parity: the HIP build and query against the oracle, bit for bit.
+
+Build parity follows the reference's own parallel_build_matches_serial_build
+(src/build.rs:345-359): data, indices, nodes and boxes must be identical.  Query
+parity compares distances and indices exactly (assert_array_equal), which is
+stricter than the reference's SciPy check (tests/test_kdtree.py:88-89).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from kdtree import KDTree
+from oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+
+def hashed(n: int, d: int) -> np.ndarray:
+    # The data of parallel_build_matches_serial_build, src/build.rs:348-350:
+    # many repeated coordinates, so the pivot-equal fix-up runs.
+    i = np.arange(n * d, dtype=np.uint64)
+    return ((i * np.uint64(2_654_435_761)) % np.uint64(10_007)).astype(np.float64).reshape(n, d)
+
+
+BUILD_CASES = {
+    "uniform-1000x3-ls32": (lambda: np.random.default_rng(0).uniform(size=(1000, 3)), 32),
+    "hashed-2000x5-ls8": (lambda: hashed(2000, 5), 8),
+    "c1-10000x3-ls32": (lambda: np.random.default_rng(0).uniform(size=(10_000, 3)), 32),
+    "normal-5000x8-ls16": (lambda: np.random.default_rng(3).normal(size=(5000, 8)), 16),
+    "lattice-3000x2-ls1": (lambda: np.random.default_rng(4).integers(0, 7, size=(3000, 2)).astype(float), 1),
+    "uniform-100000x3-ls32": (lambda: np.random.default_rng(5).uniform(size=(100_000, 3)), 32),
+    "hashed-60000x4-ls6": (lambda: hashed(60_000, 4), 6),
+    "normal-20000x16-ls64": (lambda: np.random.default_rng(6).normal(size=(20_000, 16)), 64),
+    "line-513x1-ls6": (lambda: np.random.default_rng(7).normal(size=(513, 1)), 6),
+    "single-point": (lambda: np.array([[1.5, -2.0]]), 32),
+    "all-equal-300x3": (lambda: np.ones((300, 3)), 4),
+    "signed-zeros": (lambda: np.array([[0.0], [-0.0]] * 50), 3),
+    "wide-200x33-ls5": (lambda: np.random.default_rng(8).normal(size=(200, 33)), 5),
+}
+
+
+def assert_same_tree(gpu, ora):
+    gd, gi, gn, gb = gpu
+    od, oi, on, ob = ora
+    np.testing.assert_array_equal(gi, oi)
+    assert gd.tobytes() == od.tobytes(), "tree-order rows differ"
+    assert len(gn) == len(on)
+    for field in ("is_leaf", "a", "b", "split_dim", "order_by_box"):
+        np.testing.assert_array_equal(gn[field], on[field], err_msg=field)
+    inner = on["is_leaf"] == 0
+    assert gn["split_value"][inner].tobytes() == on["split_value"][inner].tobytes()
+    # Boxes: equal as numbers (a box may hold -0.0 where the other holds +0.0;
+    # no query arithmetic can tell them apart).
+    np.testing.assert_array_equal(gb, ob)
+
+
+@pytest.mark.parametrize("case", BUILD_CASES.values(), ids=BUILD_CASES.keys())
+def test_build_matches_oracle(case):
+    make, leafsize = case
+    data = make()
+    print(f"build case n={data.shape[0]} d={data.shape[1]} leafsize={leafsize}", flush=True)
+    tree = KDTree(data, leafsize=leafsize)
+    assert_same_tree(tree._export(), OracleTree(data, leafsize).export())
+    np.testing.assert_array_equal(tree.data, data)
+
+
+def _query_case(data, queries, k, p=2.0, max_distance=None, eps=0.0, leafsize=32, fp32=False):
+    print(f"query case n={data.shape} q={queries.shape} k={k} p={p} md={max_distance} eps={eps} "
+          f"ls={leafsize} fp32={fp32}", flush=True)
+    tree = KDTree(data, leafsize=leafsize)
+    gd, gi = tree.query(queries, k=k, p=p, max_distance=max_distance, eps=eps, fp32_screen=fp32)
+    od, oi = OracleTree(data, leafsize).query(queries, k=k, p=p, max_distance=max_distance, eps=eps)
+    np.testing.assert_array_equal(gi, oi)
+    if np.isfinite(p) and p not in (1.0, 2.0):
+        # General p: device pow versus host libm, parity at 1e-12.
+        np.testing.assert_allclose(gd, od, rtol=1e-12, atol=1e-12)
+    else:
+        assert gd.tobytes() == od.tobytes(), "distances differ in bits"
+
+
+def test_c1_configuration():
+    # n=10,000 d=3 U[0,1) seed 0, 1,000 queries seed 1, leafsize 32, k=8.
+    data = np.random.default_rng(0).uniform(size=(10_000, 3))
+    queries = np.random.default_rng(1).uniform(size=(1_000, 3))
+    _query_case(data, queries, k=8)
+
+
+@pytest.mark.parametrize("dims", list(range(1, 18)) + [24, 33])
+@pytest.mark.parametrize("p", [1.0, 2.0, np.inf])
+def test_row_widths(dims, p):
+    # row-width-NN cases of tests/test_kdtree.py:153-163.
+    data = np.random.default_rng(dims).normal(size=(512, dims))
+    queries = np.random.default_rng(1_000 + dims).normal(size=(64, dims))
+    _query_case(data, queries, k=3, p=p, leafsize=6)
+
+
+@pytest.mark.parametrize("k", [1, 2, 7, 16, 17, 32, 33, 64, 65, 200])
+def test_k_sizes(k):
+    data = np.random.default_rng(11).normal(size=(3000, 3))
+    queries = np.random.default_rng(12).normal(size=(200, 3))
+    _query_case(data, queries, k=k, leafsize=16)
+
+
+@pytest.mark.parametrize("leafsize", [1, 3, 8, 31, 32, 64, 100])
+@pytest.mark.parametrize("dims", [4, 8])
+def test_leafsizes_partial_chunks(leafsize, dims):
+    # d4 / d8 leaf tails use a different summation order (src/kernel.rs:456).
+    data = np.random.default_rng(20 + leafsize).normal(size=(2500, dims))
+    queries = np.random.default_rng(21).normal(size=(100, dims))
+    _query_case(data, queries, k=5, leafsize=leafsize)
+
+
+@pytest.mark.parametrize("p", [1.0, 2.0, 3.0, np.inf])
+def test_general_metrics(p):
+    data = np.random.default_rng(30).normal(size=(4000, 5))
+    queries = np.random.default_rng(31).normal(size=(100, 5))
+    _query_case(data, queries, k=6, p=p)
+
+
+@pytest.mark.parametrize("max_distance", [0.0, 0.05, 0.3, 1.0])
+def test_max_distance(max_distance):
+    data = np.random.default_rng(40).normal(size=(3000, 3))
+    queries = np.random.default_rng(41).normal(size=(150, 3))
+    _query_case(data, queries, k=10, max_distance=max_distance)
+
+
+@pytest.mark.parametrize("eps", [0.0, 0.1, 1.0])
+def test_eps_visit_order(eps):
+    # With eps > 0 the result depends on the visit order; the device walks the
+    # tree in the reference's order, so it still matches.
+    data = np.random.default_rng(50).normal(size=(5000, 3))
+    queries = np.random.default_rng(51).normal(size=(200, 3))
+    _query_case(data, queries, k=4, eps=eps)
+
+
+def test_ties_lattice():
+    # Integer lattice: many exactly equal distances; ties go to the smaller
+    # original index (src/query.rs:415, :465, :474).
+    g = np.stack(np.meshgrid(np.arange(12.0), np.arange(12.0), np.arange(6.0)), -1).reshape(-1, 3)
+    data = np.concatenate([g, g[::7]])  # exact duplicates too
+    queries = np.random.default_rng(60).integers(0, 12, size=(300, 3)).astype(float) + 0.5 * (
+        np.random.default_rng(61).integers(0, 2, size=(300, 3))
+    )
+    _query_case(data, queries, k=9, leafsize=8)
+
+
+def test_order_by_box_data():
+    # Points on a 1-D curve in 3-D: boxes shrink off-axis, so order_by_box fires.
+    t = np.random.default_rng(70).uniform(0, 10, size=6000)
+    data = np.stack([t, np.sin(t), np.cos(t) * 0.01], 1)
+    tree = KDTree(data, leafsize=8)
+    assert tree._export()[2]["order_by_box"].sum() > 0
+    queries = np.random.default_rng(71).uniform(-1, 11, size=(300, 3))
+    _query_case(data, queries, k=5, leafsize=8)
+    _query_case(data, queries, k=5, leafsize=8, eps=0.5)
+
+
+def test_fp32_screen_is_exact():
+    rng = np.random.default_rng(80)
+    centres = rng.uniform(0, 10, size=(16, 8))
+    data = centres[rng.integers(0, 16, size=20000)] + 0.3 * rng.normal(size=(20000, 8))
+    queries = centres[rng.integers(0, 16, size=500)] + 0.3 * rng.normal(size=(500, 8))
+    _query_case(data, queries, k=32, fp32=True)
+    _query_case(data[:, :3].copy(), queries[:, :3].copy(), k=16, fp32=True)
+
+
+def test_large_query_batch():
+    data = np.random.default_rng(90).uniform(size=(200_000, 3))
+    queries = np.random.default_rng(91).uniform(size=(20_000, 3))
+    _query_case(data, queries, k=16)

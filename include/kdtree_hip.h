@@ -51,6 +51,11 @@ enum kdt_error {
 /* Query flags. */
 #define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 leaf screen with a proven error margin,
                                       FP64 recheck: identical results, fewer bytes */
+#define KDT_QUERY_ASSUME_FINITE 0x2u /* kdt_query_device only: the caller has already
+                                        checked that every query coordinate is finite
+                                        (e.g. by an earlier call on the same buffer), so
+                                        the call skips the check, allocates nothing and
+                                        returns without synchronising the stream */
 
 /* Node record exported for inspection (24 bytes, the reference's Node budget,
  * src/tree.rs:97-100).  Leaf: a = first row, b = end row.  Inner: a = left child
@@ -85,9 +90,10 @@ int kdt_query(const kdt_tree* tree, const double* queries, uint64_t n_queries, u
               uint64_t k, double p, double max_distance, double eps, uint32_t flags,
               double* out_dist, int64_t* out_idx);
 
-/* Same, with queries and outputs in device memory; runs on `stream` and returns
- * without synchronising unless validation needs the device (non-finite queries
- * are reported by this call, which then synchronises `stream`). */
+/* Same, with queries and outputs in device memory, on `stream`.  The finiteness
+ * check of the queries runs on the device and this call synchronises `stream` to
+ * report it, unless flags holds KDT_QUERY_ASSUME_FINITE, in which case the call
+ * only enqueues the query kernel and returns. */
 int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_queries,
                      uint64_t ndim, uint64_t k, double p, double max_distance, double eps,
                      uint32_t flags, double* d_out_dist, int64_t* d_out_idx, void* stream);

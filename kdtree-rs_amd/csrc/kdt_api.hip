@@ -130,6 +130,13 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   qp.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   std::string smsg;
   const int scalar_rc = check_scalars(p, max_distance, eps, qp, smsg);
+  if (q_on_device && out_on_device && (flags & KDT_QUERY_ASSUME_FINITE)) {
+    if (scalar_rc) return fail(scalar_rc, smsg);
+    qp.nq = nq;
+    std::string msg;
+    const int rc = kdt::launch_query(t, qp, q, out_d, out_i, t.err, s, msg);
+    return rc ? fail(rc, msg) : KDT_OK;
+  }
 
   // Queries per launch: bounded so that device buffers stay modest for huge
   // host batches.  Device-pointer calls run in one launch.
@@ -317,7 +324,7 @@ int kdt_export(const kdt_tree* tree, double* data, uint32_t* indices, kdt_node_r
 void kdt_free(kdt_tree* tree) {
   if (!tree) return;
   kdt::Tree& t = tree->t;
-  void* ptrs[] = {t.nodes, t.boxes, t.pts, t.pts32, t.idx, t.rows};
+  void* ptrs[] = {t.nodes, t.boxes, t.pts, t.pts32, t.idx, t.rows, t.err};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete tree;

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(BT) void k_part_count(const uint64_t* __restrict__ 
 // Per node: exclusive scan of per-chunk counts (npair values per chunk).
 __global__ void k_scan_chunks(const LvlNode* __restrict__ lvl, uint32_t nslots, uint32_t leafsize,
                               const uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, int npair,
-                              Sel* __restrict__ sel, int* err) {
+                              Sel* __restrict__ sel, int* err, int* max_need) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= nslots) return;
   const LvlNode nd = lvl[slot];
@@ -394,6 +394,8 @@ __global__ void k_scan_chunks(const LvlNode* __restrict__ lvl, uint32_t nslots, 
   if (npair == 2) {
     sel[slot].m = acc0;
     if (acc0 != acc1) atomicOr(err, 2);  // both sides misplace the same count
+    const int need = (int)(nd.len / 2 - sel[slot].lt);
+    if (need > 0) atomicMax(max_need, need);
   }
 }
 
@@ -475,11 +477,23 @@ __global__ __launch_bounds__(BT) void k_fix_count(const uint64_t* __restrict__ u
   if (threadIdx.x == 0) cnt[blockIdx.x] = ne;
 }
 
-// Fix-up: find e, the position of the (mid-lt)-th pivot-equal row.
+// Fix-up (src/build.rs:303-313), made parallel.  Let need = mid - lt and let
+// s_1 < ... < s_need be the positions of the first `need` pivot-equal rows at
+// or after lt.  The t-th swap of the loop exchanges the rows at place =
+// lt + t - 1 and scan = s_t, so the loop is the map g(lt + t - 1) = s_t,
+// applied in increasing t:
+//   * the pivot-equal row at s_t ends at lt + t - 1;
+//   * a row in [mid, s_need] that is not pivot-equal never moves;
+//   * a row at P in [lt, mid) that is not pivot-equal is carried by each swap
+//     whose place it occupies: P -> g(P) -> g(g(P)) -> ..., and stops at the
+//     first position >= mid.
+// g is increasing with g(x) > x, so the chains are followed by pointer
+// jumping.  This kernel writes g (node-relative) and `last` = s_need.
 __global__ __launch_bounds__(BT) void k_fix_find(const uint64_t* __restrict__ ukey,
                                                  const Chunk* __restrict__ chunks,
                                                  const LvlNode* __restrict__ lvl, uint32_t leafsize,
-                                                 Sel* __restrict__ sel, const uint32_t* __restrict__ off) {
+                                                 Sel* __restrict__ sel, const uint32_t* __restrict__ off,
+                                                 uint32_t* __restrict__ gmap) {
   __shared__ uint32_t s_e[4];
   const Chunk c = chunks[blockIdx.x];
   const LvlNode nd = lvl[c.slot];
@@ -494,12 +508,35 @@ __global__ __launch_bounds__(BT) void k_fix_find(const uint64_t* __restrict__ uk
     const bool fe = r < c.end && r - nd.start >= s.lt && from_total_key(ukey[r]) == s.pivot;
     uint32_t te;
     const uint32_t e = block_excl_count(fe, s_e, te);
-    if (fe && re + e == need - 1) sel[c.slot].last = r - nd.start;
+    if (fe && re + e < need) {
+      gmap[nd.start + s.lt + re + e] = r - nd.start;  // g(lt + t - 1) = s_t
+      if (re + e == need - 1) sel[c.slot].last = r - nd.start;
+    }
     re += te;
   }
 }
 
-// Fix-up: stable partition of the window [lt, last] into tmp.
+// One pointer-jumping round over [lt, mid): G'(x) = G(G(x)) while G(x) < mid.
+__global__ __launch_bounds__(BT) void k_fix_jump(const Chunk* __restrict__ chunks,
+                                                 const LvlNode* __restrict__ lvl, uint32_t leafsize,
+                                                 const Sel* __restrict__ sel,
+                                                 const uint32_t* __restrict__ gin,
+                                                 uint32_t* __restrict__ gout) {
+  const Chunk c = chunks[blockIdx.x];
+  const LvlNode nd = lvl[c.slot];
+  if (!is_inner(nd, leafsize)) return;
+  const uint32_t lt = sel[c.slot].lt, mid = nd.len / 2;
+  if (lt == mid) return;
+  for (uint32_t r = c.begin + threadIdx.x; r < c.end; r += BT) {
+    const uint32_t rel = r - nd.start;
+    if (rel < lt || rel >= mid) continue;
+    uint32_t v = gin[r];
+    if (v >= lt && v < mid) v = gin[nd.start + v];
+    gout[r] = v;
+  }
+}
+
+// Fix-up: scatter the window [lt, last] to its final order in tmp.
 __global__ __launch_bounds__(BT) void k_fix_scatter(const double* __restrict__ rows,
                                                     const uint32_t* __restrict__ idx,
                                                     const uint64_t* __restrict__ ukey, uint32_t D,
@@ -507,7 +544,9 @@ __global__ __launch_bounds__(BT) void k_fix_scatter(const double* __restrict__ r
                                                     const LvlNode* __restrict__ lvl, uint32_t leafsize,
                                                     const Sel* __restrict__ sel,
                                                     const uint32_t* __restrict__ off,
-                                                    double* __restrict__ trows, uint32_t* __restrict__ tidx) {
+                                                    const uint32_t* __restrict__ gfinal,
+                                                    double* __restrict__ trows, uint32_t* __restrict__ tidx,
+                                                    int* err) {
   __shared__ uint32_t s_e[4];
   const Chunk c = chunks[blockIdx.x];
   const LvlNode nd = lvl[c.slot];
@@ -524,12 +563,16 @@ __global__ __launch_bounds__(BT) void k_fix_scatter(const double* __restrict__ r
     const bool fe = inwin && from_total_key(ukey[r]) == s.pivot;
     uint32_t te;
     const uint32_t e = block_excl_count(fe, s_e, te);
-    const uint32_t erank = re + e;
-    const uint32_t dest = fe ? s.lt + erank : mid + (rel - s.lt - erank);
-    if (inwin && dest < nd.len) {
-      const uint32_t pd = nd.start + dest;
-      for (uint32_t j = 0; j < D; ++j) trows[(size_t)pd * D + j] = rows[(size_t)r * D + j];
-      tidx[pd] = idx[r];
+    if (inwin) {
+      uint32_t dest = rel;
+      if (fe) dest = s.lt + re + e;
+      else if (rel < mid) dest = gfinal[r];
+      if (dest < mid && !fe) atomicOr(err, 4);  // chain not resolved (never expected)
+      if (dest < nd.len) {
+        const uint32_t pd = nd.start + dest;
+        for (uint32_t j = 0; j < D; ++j) trows[(size_t)pd * D + j] = rows[(size_t)r * D + j];
+        tidx[pd] = idx[r];
+      }
     }
     re += te;
   }
@@ -747,14 +790,14 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   KDT_HIP_CHECK(ukey.alloc(n));
   KDT_HIP_CHECK(sel.alloc(max_slots));
   KDT_HIP_CHECK(nodes.alloc(t.n_nodes));
-  KDT_HIP_CHECK(err.alloc(1));
+  KDT_HIP_CHECK(err.alloc(2));  // [0] error bits, [1] largest fix-up need of a level
 
   KDT_HIP_CHECK(hipMemcpyAsync(d_lvl.p, all_nodes.data(), tot_nodes * sizeof(LvlNode), hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(d_chunks.p, all_chunks.data(), tot_chunks * sizeof(Chunk), hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(d_leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(rows.p, d_data, n * D * 8, hipMemcpyDeviceToDevice, s));
   KDT_HIP_CHECK(hipMemsetAsync(hist.p, 0, max_slots * 256 * 4, s));
-  KDT_HIP_CHECK(hipMemsetAsync(err.p, 0, 4, s));
+  KDT_HIP_CHECK(hipMemsetAsync(err.p, 0, 8, s));
 
   // Finiteness of the input (the root bbox check, src/build.rs:51-57).
   launch_check_finite(rows.p, n * D, err.p, s);
@@ -791,14 +834,29 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
       }
       const uint32_t sb = (ns + 255) / 256;
       k_part_count<<<nc, BT, 0, s>>>(ukey.p, C, L, lsu, sel.p, cnt.p);
-      k_scan_chunks<<<sb, 256, 0, s>>>(L, ns, lsu, cnt.p, off.p, 2, sel.p, err.p);
+      k_scan_chunks<<<sb, 256, 0, s>>>(L, ns, lsu, cnt.p, off.p, 2, sel.p, err.p, err.p + 1);
       k_part_lists<<<nc, BT, 0, s>>>(ukey.p, C, L, lsu, sel.p, off.p, alist.p, blist.p);
       k_part_swap<<<nc, BT, 0, s>>>(rows.p, idx.p, ukey.p, Du, C, L, lsu, sel.p, alist.p, blist.p);
-      k_fix_count<<<nc, BT, 0, s>>>(ukey.p, C, L, lsu, sel.p, cnt.p);
-      k_scan_chunks<<<sb, 256, 0, s>>>(L, ns, lsu, cnt.p, off.p, 1, sel.p, err.p);
-      k_fix_find<<<nc, BT, 0, s>>>(ukey.p, C, L, lsu, sel.p, off.p);
-      k_fix_scatter<<<nc, BT, 0, s>>>(rows.p, idx.p, ukey.p, Du, C, L, lsu, sel.p, off.p, trows.p, tidx.p);
-      k_fix_copy<<<nc, BT, 0, s>>>(rows.p, idx.p, Du, C, L, lsu, sel.p, trows.p, tidx.p);
+      // The fix-up only runs where pivot-equal rows must move; the level's
+      // largest need also bounds the pointer-jumping rounds.
+      int max_need = 0;
+      KDT_HIP_CHECK(hipMemcpyAsync(&max_need, err.p + 1, 4, hipMemcpyDeviceToHost, s));
+      KDT_HIP_CHECK(hipStreamSynchronize(s));
+      if (max_need > 0) {
+        KDT_HIP_CHECK(hipMemsetAsync(err.p + 1, 0, 4, s));
+        k_fix_count<<<nc, BT, 0, s>>>(ukey.p, C, L, lsu, sel.p, cnt.p);
+        k_scan_chunks<<<sb, 256, 0, s>>>(L, ns, lsu, cnt.p, off.p, 1, sel.p, err.p, err.p + 1);
+        k_fix_find<<<nc, BT, 0, s>>>(ukey.p, C, L, lsu, sel.p, off.p, alist.p);
+        uint32_t* g = alist.p;
+        uint32_t* g2 = blist.p;
+        for (int span = 1; span < max_need; span *= 2) {
+          k_fix_jump<<<nc, BT, 0, s>>>(C, L, lsu, sel.p, g, g2);
+          std::swap(g, g2);
+        }
+        k_fix_scatter<<<nc, BT, 0, s>>>(rows.p, idx.p, ukey.p, Du, C, L, lsu, sel.p, off.p, g, trows.p,
+                                         tidx.p, err.p);
+        k_fix_copy<<<nc, BT, 0, s>>>(rows.p, idx.p, Du, C, L, lsu, sel.p, trows.p, tidx.p);
+      }
     }
     k_write_nodes<<<(ns + 255) / 256, 256, 0, s>>>(L, ns, lsu, sel.p, nodes.p, node_len.p);
   }
@@ -824,6 +882,9 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   t.pts32 = pts32.release();
   t.idx = idx.release();
   t.rows = rows.release();
+  KDT_HIP_CHECK(hipMalloc(&t.err, sizeof(int)));
+  KDT_HIP_CHECK(hipMemsetAsync(t.err, 0, sizeof(int), s));
+  KDT_HIP_CHECK(hipStreamSynchronize(s));
   return 0;
 }
 

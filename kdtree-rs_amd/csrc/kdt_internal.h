@@ -45,6 +45,7 @@ struct Tree {
   uint32_t* idx = nullptr;
   double* rows = nullptr;
   double max_abs = 0.0;  // max |coordinate| over the data (root box)
+  int* err = nullptr;     // device int for kernel error bits of unchecked calls
 };
 
 // Build on the device from row-major f64 data already in device memory.

@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
+                    help="c2 (default, the metric's configuration); c3 precision mode; c4 multi-tree batch")
     return ap.parse_args()
 
 
@@ -65,6 +67,10 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the workload is a single-GPU "
               "query pass and no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        return run_c3(args)
+    if args.workload == "c4":
+        return run_c4(args)
     import torch
 
     lib = load_library()
@@ -177,6 +183,123 @@ def main():
     }
     lib.kdt_free(tree)
     print(json.dumps(line), flush=True)
+
+
+def _time_steps(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_c3(args):
+    """Precision mode: n = 2^22, d = 8, 64-component Gaussian mixture, 10^6
+    queries, k = 32; all-FP64 leaf scan versus FP32 screen + FP64 recheck."""
+    import torch
+
+    from kdtree import forest as F
+    from kdtree.workloads import C3, c3_inputs
+    from oracle import OracleTree
+
+    data_h, q_h = c3_inputs()
+    k, ls = C3["k"], C3["leafsize"]
+    data = torch.from_numpy(data_h).cuda()
+    q = torch.from_numpy(q_h).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree = F.build(data, ls)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    nq = q.shape[0]
+    od64, oi64 = F.query_device(tree, q, k, flags=0)
+    od32, oi32 = F.query_device(tree, q, k, flags=F.KDT_QUERY_FP32_SCREEN)
+    s64 = _time_steps(lambda: F.query_device(tree, q, k, od64, oi64), args.steps, args.warmup)
+    s32 = _time_steps(lambda: F.query_device(tree, q, k, od32, oi32,
+                                             flags=F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN),
+                      args.steps, args.warmup)
+    same_idx = bool(torch.equal(oi64, oi32))
+    d64, d32 = od64.cpu().numpy(), od32.cpu().numpy()
+    rel = float(np.max(np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-300)))
+    sample = 2000
+    otree = OracleTree(data_h, ls)
+    od, oi = otree.query(q_h[:sample], k=k)
+    parity = bool(np.array_equal(oi, oi64[:sample].cpu().numpy()) and od.tobytes() == d64[:sample].tobytes())
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        m = 20_000
+        t0 = time.perf_counter()
+        otree.query(q_h[:m], k=k, threads=threads)
+        cpu = {"value": m / (time.perf_counter() - t0), "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"{m} of the {nq} queries, k={k}, {threads} threads (oracle)"}
+    F_tree = tree
+    out = {
+        "metric": "k-NN queries/s (exact indices), precision mode",
+        "config": {"workload": "knn-gmm64-n4194304-d8-q1000000-k32-leafsize32-p2"},
+        "fp64": {"value": nq / s64, "ms_per_step": s64 * 1e3},
+        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3},
+        "indices_identical": same_idx, "max_rel_distance_error_fp32_vs_fp64": rel,
+        "parity_sample_bit_exact_vs_oracle": parity, "build_ms": build_ms, "cpu_baseline": cpu,
+        "vs_cpu_fp32_screen": (nq / s32) / cpu["value"] if cpu else None,
+    }
+    F_tree.free()
+    print(json.dumps(out), flush=True)
+
+
+def run_c4(args):
+    """256 trees (n log-uniform in [1e3,1e6], d in {2,3,8,16}, float32 uniform
+    promoted to float64), per-tree query counts log-uniform in [1, 1e5], k in
+    {1, 8, 32}; leafsize autotuned over {8,16,32,64}.  One persistent launch
+    versus one launch per tree versus the per-tree launches as a HIP graph."""
+    import torch
+
+    from kdtree import forest as F
+    from kdtree.workloads import c4_specs, c4_tree_inputs
+    from oracle import OracleTree
+
+    specs = c4_specs()
+    items, host = [], []
+    t0 = time.perf_counter()
+    for t, (n, d, q, k) in enumerate(specs):
+        data, queries = c4_tree_inputs(t, n, d, q)
+        tree = F.autotune_build(data, queries[:4096], k, reps=2)
+        items.append((tree, torch.from_numpy(queries).cuda(), k))
+        if t < 8:
+            host.append((data, queries, k, tree.leafsize))
+    tune_s = time.perf_counter() - t0
+    batch = F.Batch(items)
+    nq = batch.n_queries
+    res = {}
+    for mode in ("per_tree", "graph", "persistent"):
+        res[mode] = nq / _time_steps(lambda: batch.run(mode), args.steps, args.warmup)
+    batch.run("persistent")
+    torch.cuda.synchronize()
+    parity = True
+    for j, (data, queries, k, ls) in enumerate(host):
+        od, oi = OracleTree(data, ls).query(queries, k=k)
+        gd, gi = batch.results(j)
+        parity &= bool(np.array_equal(gi.cpu().numpy(), oi) and gd.cpu().numpy().tobytes() == od.tobytes())
+    leafs = {}
+    for tree, _, _ in items:
+        leafs[tree.leafsize] = leafs.get(tree.leafsize, 0) + 1
+    out = {
+        "metric": "k-NN queries/s over a 256-tree batch",
+        "config": {"workload": "knn-256-trees-n1e3..1e6-d{2,3,8,16}-k{1,8,32}", "n_queries": nq,
+                   "n_points": int(sum(s[0] for s in specs))},
+        "queries_per_s": res,
+        "persistent_vs_per_tree": res["persistent"] / res["per_tree"],
+        "autotuned_leafsize_counts": leafs, "autotune_s": tune_s,
+        "parity_first_8_trees_bit_exact": parity,
+    }
+    for tree, _, _ in items:
+        tree.free()
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

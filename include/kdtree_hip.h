@@ -54,8 +54,27 @@ enum kdt_error {
 #define KDT_QUERY_ASSUME_FINITE 0x2u /* kdt_query_device only: the caller has already
                                         checked that every query coordinate is finite
                                         (e.g. by an earlier call on the same buffer), so
-                                        the call skips the check, allocates nothing and
-                                        returns without synchronising the stream */
+                                        the call skips the check and returns without
+                                        synchronising the stream.  Batches of 8192 or
+                                        more queries still take stream-ordered scratch
+                                        (hipMallocAsync) for the leaf-order sort. */
+
+/* Batch flags (kdt_query_batch_device). */
+#define KDT_BATCH_PER_TREE_LAUNCHES 0x100u /* one launch per item instead of one persistent launch */
+#define KDT_BATCH_NO_ORDER 0x200u          /* keep the given query order (no leaf-order sort), so
+                                              the call enqueues kernels only */
+
+/* One item of a multi-tree batch: queries (device, n_queries x ndim, ndim =
+ * the tree's) against one tree, k neighbours, outputs n_queries x k (device). */
+typedef struct kdt_batch_item {
+  const kdt_tree* tree;
+  const double* queries;
+  uint64_t n_queries;
+  uint64_t ndim;
+  uint64_t k;
+  double* out_dist;
+  int64_t* out_idx;
+} kdt_batch_item;
 
 /* Node record exported for inspection (24 bytes, the reference's Node budget,
  * src/tree.rs:97-100).  Leaf: a = first row, b = end row.  Inner: a = left child
@@ -97,6 +116,15 @@ int kdt_query(const kdt_tree* tree, const double* queries, uint64_t n_queries, u
 int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_queries,
                      uint64_t ndim, uint64_t k, double p, double max_distance, double eps,
                      uint32_t flags, double* d_out_dist, int64_t* d_out_idx, void* stream);
+
+/* Many trees and query sets in one persistent launch.  No reference
+ * counterpart (the reference answers one matrix against one tree per call,
+ * src/query.rs:29-78); each item's result is identical to kdt_query_device on
+ * that item.  The caller guarantees finite queries (as KDT_QUERY_ASSUME_FINITE);
+ * p, max_distance and eps apply to every item and are validated like
+ * kdt_query.  Runs on `stream` without synchronising. */
+int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_items, double p,
+                           double max_distance, double eps, uint32_t flags, void* stream);
 
 /* Rows in the caller's original order (n_points x ndim).  Replaces
  * Tree::original_data, src/tree.rs:60-68 (the `data` getter, src/lib.rs:138-146). */

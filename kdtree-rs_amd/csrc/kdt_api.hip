@@ -14,50 +14,21 @@
 #include "../../include/kdtree_hip.h"
 #include "kdt_internal.h"
 
-struct kdt_tree {
-  kdt::Tree t;
-};
-
 namespace kdt {
 uint64_t count_tree_nodes(uint64_t n, uint64_t leafsize);
 }
 
 namespace {
-
 thread_local std::string g_last_error;
+}
+
+void kdt::set_last_error(std::string msg) { g_last_error = std::move(msg); }
+
+namespace {
 
 int fail(int code, std::string msg) {
   g_last_error = std::move(msg);
   return code;
-}
-
-// Rust's `{}` for f64: shortest round-trip digits, never an exponent, "inf",
-// "-inf", "NaN", "-0" for negative zero, no trailing ".0" on integral values.
-std::string rust_f64(double x) {
-  if (std::isnan(x)) return "NaN";
-  if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
-  std::string out = std::signbit(x) ? "-" : "";
-  x = std::fabs(x);
-  if (x == 0.0) return out + "0";
-  char buf[64];
-  auto r = std::to_chars(buf, buf + sizeof buf, x, std::chars_format::scientific);
-  std::string sci(buf, r.ptr);
-  const size_t epos = sci.find('e');
-  std::string mant = sci.substr(0, epos);
-  const int exp10 = std::stoi(sci.substr(epos + 1));
-  std::string digits;
-  for (char c : mant)
-    if (c != '.') digits.push_back(c);
-  // value = 0.digits * 10^(exp10 + 1)
-  const int point = exp10 + 1;
-  if (point <= 0) {
-    out += "0." + std::string((size_t)(-point), '0') + digits;
-  } else if ((size_t)point >= digits.size()) {
-    out += digits + std::string((size_t)point - digits.size(), '0');
-  } else {
-    out += digits.substr(0, (size_t)point) + "." + digits.substr((size_t)point);
-  }
-  return out;
 }
 
 hipStream_t host_stream() { return hipStreamPerThread; }
@@ -81,38 +52,6 @@ struct DevMem {
     if (e_ != hipSuccess) return fail(KDT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// Scalar part of the query validation (src/query.rs:38-57) after k and shape;
-// 0 when fine.  Fills the metric and derived parameters.
-int check_scalars(double p, double max_distance, double eps, kdt::QueryParams& qp, std::string& msg) {
-  if (!std::isfinite(eps) || eps < 0.0) {
-    msg = "eps must be finite and non-negative, got " + rust_f64(eps);
-    return KDT_ERR_INVALID_EPS;
-  }
-  if (std::isnan(max_distance) || max_distance < 0.0) {
-    msg = "max_distance must be finite and non-negative, or infinity, got " + rust_f64(max_distance);
-    return KDT_ERR_INVALID_MAX_DISTANCE;
-  }
-  // Metric::new, src/metric.rs:45-59
-  int metric;
-  if (std::isinf(p) && p > 0) metric = kdt::M_LINF;
-  else if (!std::isfinite(p) || p < 1.0) {
-    msg = "p must be finite and >= 1, or infinity, got " + rust_f64(p);
-    return KDT_ERR_INVALID_METRIC;
-  } else if (p == 1.0) metric = kdt::M_L1;
-  else if (p == 2.0) metric = kdt::M_L2;
-  else metric = kdt::M_LP;
-  auto reduce = [&](double x) {  // Metric::reduce, src/metric.rs:63-71
-    if (metric == kdt::M_L2) return x * x;
-    if (metric == kdt::M_LP) return std::pow(x, p);
-    return x;
-  };
-  qp.metric = metric;
-  qp.p = p;
-  qp.cutoff = reduce(max_distance);
-  qp.eps_factor = reduce(1.0 + eps);
-  return 0;
-}
-
 int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t nq, uint64_t ndim,
                uint64_t k, double p, double max_distance, double eps, uint32_t flags, double* out_d,
                int64_t* out_i, bool out_on_device, hipStream_t s) {
@@ -129,7 +68,7 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   qp.k = k;
   qp.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   std::string smsg;
-  const int scalar_rc = check_scalars(p, max_distance, eps, qp, smsg);
+  const int scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
   if (q_on_device && out_on_device && (flags & KDT_QUERY_ASSUME_FINITE)) {
     if (scalar_rc) return fail(scalar_rc, smsg);
     qp.nq = nq;

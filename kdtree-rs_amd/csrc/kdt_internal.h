@@ -57,7 +57,18 @@ struct QueryParams {
   int metric = M_L2;
   double p = 2.0, cutoff = 0.0, eps_factor = 1.0;
   bool fp32_screen = false;
+  bool allow_order = true;  // large batches may run in leaf order
 };
+
+// Metric, cutoff and eps factor from (p, max_distance, eps) with the checks
+// and messages of src/query.rs:49-62 and src/metric.rs:45-71; 0 or an error code.
+int query_params(double p, double max_distance, double eps, QueryParams& qp, std::string& msg);
+
+// Rust's Display of an f64 (shortest round-trip digits, no exponent).
+std::string rust_f64(double x);
+
+// Thread-local message behind kdt_last_error().
+void set_last_error(std::string msg);
 
 // Launch the exact k-NN query; d_q [nq x d] f64, outputs [nq x k].
 // *d_err gets bit 2 if the kernel meets a malformed tree.
@@ -80,6 +91,11 @@ struct NodeRecord {
 static_assert(sizeof(NodeRecord) == 24, "NodeRecord matches kdt_node_record");
 
 }  // namespace kdt
+
+// The opaque handle of the C ABI.
+struct kdt_tree {
+  kdt::Tree t;
+};
 
 #define KDT_HIP_CHECK(expr)                                                      \
   do {                                                                           \

@@ -6,31 +6,103 @@
 // src/build.rs:60) have compiled kernels with the k list in registers when
 // k <= 64; everything else runs the runtime-width kernel whose k list is in LDS.
 
+#include <charconv>
 #include <cmath>
+#include <cstdlib>
 
 #include "kdt_query_kernels.h"
 
 namespace kdt {
 
+int order_queries(const Tree& t, const double* d_q, uint64_t nq, uint32_t* d_order, hipStream_t s,
+                  std::string& msg);
+
 namespace {
 constexpr size_t LDS_LIST_LIMIT = 60000;  // bytes of dynamic LDS per block
+constexpr uint64_t ORDER_MIN_QUERIES = 8192; // smaller batches run in the given order
 
 bool compiled_width(uint64_t d) { return d == 1 || d == 2 || d == 3 || d == 4 || d == 8 || d == 16; }
 
-// Lanes per query: enough for the k list and for one leaf (up to 64) per pass.
+// Lanes per query: the narrowest group that holds the k list.  Narrow groups
+// keep more queries in flight per wavefront, which measured faster than
+// covering a whole leaf per pass (a 32-row leaf takes two passes at G = 16).
+// KDT_GROUP_WIDTH (16, 32 or 64) overrides the choice when it still holds k.
 int group_width(uint64_t k, uint64_t leafsize) {
-  const uint64_t need = std::max<uint64_t>(k, std::min<uint64_t>(leafsize, 64));
-  return need <= 16 ? 16 : need <= 32 ? 32 : 64;
+  (void)leafsize;
+  int g = k <= 16 ? 16 : k <= 32 ? 32 : 64;
+  if (const char* env = std::getenv("KDT_GROUP_WIDTH")) {
+    const int want = std::atoi(env);
+    if ((want == 16 || want == 32 || want == 64) && (uint64_t)want >= k) g = want;
+  }
+  return g;
 }
 }  // namespace
 
-int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
-                 int64_t* d_idx, int* d_err, hipStream_t s, std::string& msg) {
-  KArgs a{};
+std::string rust_f64(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+  std::string out = std::signbit(x) ? "-" : "";
+  x = std::fabs(x);
+  if (x == 0.0) return out + "0";
+  char buf[64];
+  auto r = std::to_chars(buf, buf + sizeof buf, x, std::chars_format::scientific);
+  std::string sci(buf, r.ptr);
+  const size_t epos = sci.find('e');
+  std::string mant = sci.substr(0, epos);
+  const int exp10 = std::stoi(sci.substr(epos + 1));
+  std::string digits;
+  for (char c : mant)
+    if (c != '.') digits.push_back(c);
+  // value = 0.digits * 10^(exp10 + 1)
+  const int point = exp10 + 1;
+  if (point <= 0) {
+    out += "0." + std::string((size_t)(-point), '0') + digits;
+  } else if ((size_t)point >= digits.size()) {
+    out += digits + std::string((size_t)point - digits.size(), '0');
+  } else {
+    out += digits.substr(0, (size_t)point) + "." + digits.substr((size_t)point);
+  }
+  return out;
+}
+
+int query_params(double p, double max_distance, double eps, QueryParams& qp, std::string& msg) {
+  if (!std::isfinite(eps) || eps < 0.0) {
+    msg = "eps must be finite and non-negative, got " + rust_f64(eps);
+    return 5;
+  }
+  if (std::isnan(max_distance) || max_distance < 0.0) {
+    msg = "max_distance must be finite and non-negative, or infinity, got " + rust_f64(max_distance);
+    return 6;
+  }
+  // Metric::new, src/metric.rs:45-59
+  int metric;
+  if (std::isinf(p) && p > 0) metric = M_LINF;
+  else if (!std::isfinite(p) || p < 1.0) {
+    msg = "p must be finite and >= 1, or infinity, got " + rust_f64(p);
+    return 4;
+  } else if (p == 1.0) metric = M_L1;
+  else if (p == 2.0) metric = M_L2;
+  else metric = M_LP;
+  auto reduce = [&](double x) {  // Metric::reduce, src/metric.rs:63-71
+    if (metric == M_L2) return x * x;
+    if (metric == M_LP) return std::pow(x, p);
+    return x;
+  };
+  qp.metric = metric;
+  qp.p = p;
+  qp.cutoff = reduce(max_distance);
+  qp.eps_factor = reduce(1.0 + eps);
+  return 0;
+}
+
+int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out, int64_t* d_idx,
+                int* d_err, KArgs& a) {
+  a = KArgs{};
   a.nodes = t.nodes;
   a.boxes = t.boxes;
   a.pts = t.pts;
   a.pts32 = t.pts32;
+  a.rows = t.rows;
   a.idx = t.idx;
   a.q = d_q;
   a.out_d = d_out;
@@ -51,7 +123,32 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
   a.screen_gam = (1.0 + D * u / (1.0 - D * u)) * (1.0 + 1e-14);
   a.screen_shrink = 1.0 + 2.0 * (D + 3.0) * std::ldexp(1.0, -53);
   a.max_abs = t.max_abs;
+  return 0;
+}
+
+int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
+                 int64_t* d_idx, int* d_err, hipStream_t s, std::string& msg) {
+  KArgs a;
+  build_kargs(t, qp, d_q, d_out, d_idx, d_err, a);
   if (qp.nq == 0) return 0;
+
+  // Large batches run grouped by leaf (see kdt_order.hip).
+  uint32_t* order = nullptr;
+  const char* no_order = std::getenv("KDT_NO_QUERY_ORDER");
+  if (qp.allow_order && qp.nq >= ORDER_MIN_QUERIES && qp.nq <= 0xFFFFFFFFull &&
+      !(no_order && *no_order == '1')) {
+    hipError_t e = hipMallocAsync((void**)&order, qp.nq * 4, s);
+    if (e != hipSuccess) {
+      msg = std::string("query ordering: ") + hipGetErrorString(e);
+      return 100;
+    }
+    const int rc = order_queries(t, d_q, qp.nq, order, s, msg);
+    if (rc) {
+      (void)hipFreeAsync(order, s);
+      return rc;
+    }
+    a.order = order;
+  }
 
   const bool reg_ok = qp.metric != M_LP && compiled_width(t.d) && qp.k <= 64;
   int rc = -1;
@@ -71,6 +168,7 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
     }
     rc = launch_knn_lds(a, qp.metric, s);
   }
+  if (order) (void)hipFreeAsync(order, s);
   if (rc != 0) {
     msg = "no kernel for this query configuration";
     return 101;

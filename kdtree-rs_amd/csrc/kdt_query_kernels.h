@@ -53,8 +53,10 @@ struct KArgs {
   const double* __restrict__ boxes;
   const double* __restrict__ pts;
   const float* __restrict__ pts32;
+  const double* __restrict__ rows;     // tree-order rows, row-major
   const uint32_t* __restrict__ idx;
   const double* __restrict__ q;
+  const uint32_t* __restrict__ order;  // processing order of the queries (null: as given)
   double* __restrict__ out_d;
   int64_t* __restrict__ out_i;
   int* err;             // set to 2 on a malformed tree (never on a built one)
@@ -148,14 +150,14 @@ __device__ __forceinline__ double box_dist_d(const double* q, const double* __re
   }
 }
 
-// Reduced distance from q to point o of a leaf bucket of L rows
-// (coordinate c at base[c*L + o]), in the reference's order for width D.
+// Reduced distance from q to one point whose coordinate c is at pt[c * sc]
+// (sc = L in a coordinate-major leaf bucket, 1 in a row), in the reference's
+// order for width D; `full` = the point lies in a complete chunk of its leaf.
 template <int M, int DT>
-__device__ __forceinline__ double point_dist_d(const double* q, const double* __restrict__ base,
-                                               uint32_t L, uint32_t o, uint32_t Drt, bool full,
-                                               double p) {
+__device__ __forceinline__ double point_dist_d(const double* q, const double* __restrict__ pt,
+                                               uint32_t sc, uint32_t Drt, bool full, double p) {
   const uint32_t D = DT > 0 ? DT : Drt;
-  auto term = [&](uint32_t c) -> double { return axis_term<M>(base[(size_t)c * L + o] - q[c], p); };
+  auto term = [&](uint32_t c) -> double { return axis_term<M>(pt[(size_t)c * sc] - q[c], p); };
   if constexpr (M == M_LINF) {
     double m = 0.0;
 #pragma unroll
@@ -323,181 +325,188 @@ struct LdsList {
 // SCREEN: FP32 leaf screen + FP64 recheck (L2, DT > 0 only).  LDSLIST: k list
 // in LDS (G must be 64, one group per block; dynamic LDS holds the list and,
 // for DT == 0, the cell-bound axes).
+// One query: the descent of Descent::run (src/query.rs:223-232) for query row
+// qi, by the calling group of G lanes; writes row qi of the outputs.
+template <int DT, int G, int M, bool SCREEN, bool LDSLIST, typename List>
+__device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best, double* axes, double* s_q,
+                                        double* s_sa, double* s_sb, uint32_t* s_sn, uint32_t* s_sm) {
+  constexpr int DL = DT > 0 ? DT : 1;
+  const uint32_t lane = threadIdx.x & (G - 1);
+  const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const uint32_t D = DT > 0 ? DT : a.D;
+  const double eps = a.eps_factor;
+  const double* qg = a.q + qi * D;
+  double qr[DL];
+  float q32[DL];
+  const double* q;
+  double qmax = 0.0;
+  if constexpr (DT > 0) {
+#pragma unroll
+    for (int c = 0; c < DT; ++c) {
+      qr[c] = qg[c];
+      s_q[c] = qr[c];
+      axes[c] = 0.0;
+      q32[c] = __double2float_rn(qr[c]);
+      qmax = fmax(qmax, fabs(qr[c]));
+    }
+    q = qr;
+  } else {
+    q = qg;
+    for (uint32_t c = 0; c < D; ++c) axes[c] = 0.0;
+  }
+  (void)q32;
+  best.reset();
+  double bound = a.cutoff;  // BestK::bound with an empty list
+  float t32 = INFINITY;
+  double screen_e = 0.0;
+  bool screen_on = false;
+  if constexpr (SCREEN) {
+    screen_on = qmax <= 1e18 && a.max_abs <= 1e18;
+    screen_e = a.screen_e * fmax(qmax, a.max_abs) + 1e-43;
+    t32 = screen_threshold(bound, screen_e, a);
+  }
+
+  // CellBound::start / seed, src/query.rs:170-187
+  double cell = 0.0;
+  if (box_dist_d<M, DT>(q, a.boxes, D, a.p) != 0.0) {
+    for (uint32_t c = 0; c < D; ++c) {
+      const double ax = reduce_d<M>(axis_offset_d(q[c], a.boxes[c], a.boxes[D + c]), a.p);
+      axes[c] = ax;
+      cell = fold_d<M>(cell, ax);
+    }
+  }
+  int sp = 0;
+  uint32_t node = 0;
+  bool active = cell * eps <= bound;  // Descent::run, src/query.rs:227-229
+  while (active) {
+    const DevNode nd = a.nodes[node];
+    const bool leaf = (nd.b & LEAF_BIT) != 0;
+    // Preorder ids grow downwards and leaves lie inside the point range;
+    // anything else ends the query instead of walking off the arrays.
+    if (leaf ? ((uint64_t)__double_as_longlong(nd.split) > a.n_points ||
+                (uint64_t)__double_as_longlong(nd.split) < nd.a)
+             : (nd.a <= node || nd.a >= a.n_nodes || (nd.b & DIM_MASK) >= D)) {
+      if (lane == 0) atomicOr(a.err, 2);
+      break;
+    }
+    if (!leaf) {
+      // descend, src/query.rs:255-287
+      const uint32_t dim = nd.b & DIM_MASK;
+      const double qd = DT > 0 ? s_q[dim] : qg[dim];
+      const double diff = qd - nd.split;
+      const bool goleft = diff <= 0.0;
+      const uint32_t near = goleft ? node + 1 : nd.a;
+      const uint32_t far = goleft ? nd.a : node + 1;
+      const double far_axis = reduce_d<M>(fabs(diff), a.p);
+      const double old = axes[dim];
+      const double far_dist = replace_axis_d<M>(cell, old, far_axis);
+      if (nd.b & OBB_BIT) {
+        // descend_by_box, src/query.rs:298-327
+        const double nb = box_dist_d<M, DT>(q, a.boxes + (size_t)near * 2 * D, D, a.p);
+        const double fb = box_dist_d<M, DT>(q, a.boxes + (size_t)far * 2 * D, D, a.p);
+        if (nb <= fb) {
+          s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_BOXFAR | (dim << 2);
+          ++sp;
+          if (nb * eps <= bound) { node = near; continue; }
+        } else {
+          s_sa[sp] = cell; s_sb[sp] = 0.0; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;
+          ++sp;
+          if (fb * eps <= bound) {  // enter_far, src/query.rs:329-341
+            s_sa[sp] = old; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
+            ++sp;
+            axes[dim] = far_axis;
+            node = far;
+            cell = far_dist;
+            continue;
+          }
+        }
+      } else {
+        s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_FAR | (dim << 2);
+        ++sp;
+        node = near;
+        continue;
+      }
+    } else {
+      // scan_leaf, src/query.rs:343-353; leaf scans, src/kernel.rs:359-563
+      const uint32_t start = nd.a;
+      const uint32_t L = (uint32_t)__double_as_longlong(nd.split) - start;
+      const double* base = a.pts + (size_t)start * D;
+      const uint32_t full = D == 4 ? (L & ~1u) : D == 8 ? (L & ~7u) : L;
+      for (uint32_t c0 = 0; c0 < L; c0 += G) {
+        const uint32_t o = c0 + lane;
+        double dist = INFINITY;
+        bool hit = false;
+        if (o < L) {
+          bool cand = true;
+          if constexpr (SCREEN) {
+            if (screen_on)
+              cand = point_dist32<DL>(q32, a.pts32 + (size_t)start * D, L, o) <= t32;
+          }
+          if (cand) {
+            dist = point_dist_d<M, DT>(q, base + o, L, D, o < full, a.p);
+            hit = dist <= bound;
+          }
+        }
+        uint64_t m = (__ballot(hit) >> gshift) & gmask;
+        if (m) {
+          const uint32_t myidx = hit ? a.idx[start + o] : 0u;
+          do {
+            const int f = __ffsll((unsigned long long)m) - 1;
+            const double df = __shfl(dist, f, G);
+            const uint32_t xf = __shfl(myidx, f, G);
+            bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
+            m &= m - 1;
+            m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
+          } while (m);
+          if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
+        }
+      }
+    }
+    // Unwind to the next deferred visit.
+    active = false;
+    while (sp > 0) {
+      --sp;
+      const uint32_t meta = s_sm[sp], en = s_sn[sp];
+      const double ea = s_sa[sp], eb = s_sb[sp];
+      const uint32_t kind = meta & 3u, edim = meta >> 2;
+      if (kind == KIND_RESTORE) { axes[edim] = ea; continue; }
+      if (kind == KIND_BOXNEAR) {
+        if (box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound) {
+          node = en; cell = ea; active = true; break;
+        }
+        continue;
+      }
+      bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
+      if (ok) ok = box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound;
+      if (ok) {
+        s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
+        ++sp;
+        axes[edim] = ea;
+        node = en;
+        cell = eb;
+        active = true;
+        break;
+      }
+    }
+  }
+  best.template write<M>(qi, lane, a);
+}
+
 template <int DT, int G, int M, bool SCREEN, bool LDSLIST>
 __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s_q, double* s_sa,
                                          double* s_sb, uint32_t* s_sn, uint32_t* s_sm,
                                          double* l_d, uint32_t* l_i) {
   constexpr int GPB = 256 / G;
-  constexpr int DL = DT > 0 ? DT : 1;
-  const uint32_t lane = threadIdx.x & (G - 1);
   const uint32_t grp = threadIdx.x / G;
-  const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
-  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-  const uint32_t D = DT > 0 ? DT : a.D;
   const uint32_t gpb = LDSLIST ? 1u : (uint32_t)GPB;
   const uint64_t gid = (uint64_t)blockIdx.x * gpb + grp;
   const uint64_t ngroups = (uint64_t)gridDim.x * gpb;
-  const double eps = a.eps_factor;
-
   typename std::conditional<LDSLIST, LdsList, RegList<G>>::type best;
   if constexpr (LDSLIST) { best.ld = l_d; best.li = l_i; }
-
-  for (uint64_t qi = gid; qi < a.nq; qi += ngroups) {
-    const double* qg = a.q + qi * D;
-    double qr[DL];
-    float q32[DL];
-    const double* q;
-    double qmax = 0.0;
-    if constexpr (DT > 0) {
-#pragma unroll
-      for (int c = 0; c < DT; ++c) {
-        qr[c] = qg[c];
-        s_q[c] = qr[c];
-        axes[c] = 0.0;
-        q32[c] = __double2float_rn(qr[c]);
-        qmax = fmax(qmax, fabs(qr[c]));
-      }
-      q = qr;
-    } else {
-      q = qg;
-      for (uint32_t c = 0; c < D; ++c) axes[c] = 0.0;
-    }
-    (void)q32;
-    best.reset();
-    double bound = a.cutoff;  // BestK::bound with an empty list
-    float t32 = INFINITY;
-    double screen_e = 0.0;
-    bool screen_on = false;
-    if constexpr (SCREEN) {
-      screen_on = qmax <= 1e18 && a.max_abs <= 1e18;
-      screen_e = a.screen_e * fmax(qmax, a.max_abs) + 1e-43;
-      t32 = screen_threshold(bound, screen_e, a);
-    }
-
-    // CellBound::start / seed, src/query.rs:170-187
-    double cell = 0.0;
-    if (box_dist_d<M, DT>(q, a.boxes, D, a.p) != 0.0) {
-      for (uint32_t c = 0; c < D; ++c) {
-        const double ax = reduce_d<M>(axis_offset_d(q[c], a.boxes[c], a.boxes[D + c]), a.p);
-        axes[c] = ax;
-        cell = fold_d<M>(cell, ax);
-      }
-    }
-    int sp = 0;
-    uint32_t node = 0;
-    bool active = cell * eps <= bound;  // Descent::run, src/query.rs:227-229
-    while (active) {
-      const DevNode nd = a.nodes[node];
-      const bool leaf = (nd.b & LEAF_BIT) != 0;
-      // Preorder ids grow downwards and leaves lie inside the point range;
-      // anything else ends the query instead of walking off the arrays.
-      if (leaf ? ((uint64_t)__double_as_longlong(nd.split) > a.n_points ||
-                  (uint64_t)__double_as_longlong(nd.split) < nd.a)
-               : (nd.a <= node || nd.a >= a.n_nodes || (nd.b & DIM_MASK) >= D)) {
-        if (lane == 0) atomicOr(a.err, 2);
-        break;
-      }
-      if (!leaf) {
-        // descend, src/query.rs:255-287
-        const uint32_t dim = nd.b & DIM_MASK;
-        const double qd = DT > 0 ? s_q[dim] : qg[dim];
-        const double diff = qd - nd.split;
-        const bool goleft = diff <= 0.0;
-        const uint32_t near = goleft ? node + 1 : nd.a;
-        const uint32_t far = goleft ? nd.a : node + 1;
-        const double far_axis = reduce_d<M>(fabs(diff), a.p);
-        const double old = axes[dim];
-        const double far_dist = replace_axis_d<M>(cell, old, far_axis);
-        if (nd.b & OBB_BIT) {
-          // descend_by_box, src/query.rs:298-327
-          const double nb = box_dist_d<M, DT>(q, a.boxes + (size_t)near * 2 * D, D, a.p);
-          const double fb = box_dist_d<M, DT>(q, a.boxes + (size_t)far * 2 * D, D, a.p);
-          if (nb <= fb) {
-            s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_BOXFAR | (dim << 2);
-            ++sp;
-            if (nb * eps <= bound) { node = near; continue; }
-          } else {
-            s_sa[sp] = cell; s_sb[sp] = 0.0; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;
-            ++sp;
-            if (fb * eps <= bound) {  // enter_far, src/query.rs:329-341
-              s_sa[sp] = old; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
-              ++sp;
-              axes[dim] = far_axis;
-              node = far;
-              cell = far_dist;
-              continue;
-            }
-          }
-        } else {
-          s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_FAR | (dim << 2);
-          ++sp;
-          node = near;
-          continue;
-        }
-      } else {
-        // scan_leaf, src/query.rs:343-353; leaf scans, src/kernel.rs:359-563
-        const uint32_t start = nd.a;
-        const uint32_t L = (uint32_t)__double_as_longlong(nd.split) - start;
-        const double* base = a.pts + (size_t)start * D;
-        const uint32_t full = D == 4 ? (L & ~1u) : D == 8 ? (L & ~7u) : L;
-        for (uint32_t c0 = 0; c0 < L; c0 += G) {
-          const uint32_t o = c0 + lane;
-          double dist = INFINITY;
-          bool hit = false;
-          if (o < L) {
-            bool cand = true;
-            if constexpr (SCREEN) {
-              if (screen_on)
-                cand = point_dist32<DL>(q32, a.pts32 + (size_t)start * D, L, o) <= t32;
-            }
-            if (cand) {
-              dist = point_dist_d<M, DT>(q, base, L, o, D, o < full, a.p);
-              hit = dist <= bound;
-            }
-          }
-          uint64_t m = (__ballot(hit) >> gshift) & gmask;
-          if (m) {
-            const uint32_t myidx = hit ? a.idx[start + o] : 0u;
-            do {
-              const int f = __ffsll((unsigned long long)m) - 1;
-              const double df = __shfl(dist, f, G);
-              const uint32_t xf = __shfl(myidx, f, G);
-              bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
-              m &= m - 1;
-              m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
-            } while (m);
-            if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
-          }
-        }
-      }
-      // Unwind to the next deferred visit.
-      active = false;
-      while (sp > 0) {
-        --sp;
-        const uint32_t meta = s_sm[sp], en = s_sn[sp];
-        const double ea = s_sa[sp], eb = s_sb[sp];
-        const uint32_t kind = meta & 3u, edim = meta >> 2;
-        if (kind == KIND_RESTORE) { axes[edim] = ea; continue; }
-        if (kind == KIND_BOXNEAR) {
-          if (box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound) {
-            node = en; cell = ea; active = true; break;
-          }
-          continue;
-        }
-        bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
-        if (ok) ok = box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound;
-        if (ok) {
-          s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
-          ++sp;
-          axes[edim] = ea;
-          node = en;
-          cell = eb;
-          active = true;
-          break;
-        }
-      }
-    }
-    best.template write<M>(qi, lane, a);
+  for (uint64_t qs = gid; qs < a.nq; qs += ngroups) {
+    const uint64_t qi = a.order ? (uint64_t)a.order[qs] : qs;
+    knn_one<DT, G, M, SCREEN, LDSLIST>(a, qi, best, axes, s_q, s_sa, s_sb, s_sn, s_sm);
   }
 }
 

@@ -1,0 +1,192 @@
+"""Many resident trees on one GPU: autotuned builds and batched queries.
+
+No reference counterpart (the reference holds one tree per KDTree object and
+answers one query matrix per call); every result here is identical to a
+single-tree query of the same tree.  Device memory and streams come from
+PyTorch; the work is kdt_build_device / kdt_query_device /
+kdt_query_batch_device of libkdtree_hip.so.
+
+Three ways to run a batch of (tree, queries, k) items:
+  persistent  one kdt_query_batch_device launch: groups of lanes take queries
+              of every tree from one counter (default)
+  per_tree    one launch per item (the baseline)
+  graph       the per-tree launches captured once in a HIP graph and replayed
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._core import load_library
+
+KDT_QUERY_FP32_SCREEN = 0x1
+KDT_QUERY_ASSUME_FINITE = 0x2
+KDT_BATCH_PER_TREE_LAUNCHES = 0x100
+KDT_BATCH_NO_ORDER = 0x200
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("tree", ctypes.c_void_p),
+        ("queries", ctypes.c_void_p),
+        ("n_queries", ctypes.c_uint64),
+        ("ndim", ctypes.c_uint64),
+        ("k", ctypes.c_uint64),
+        ("out_dist", ctypes.c_void_p),
+        ("out_idx", ctypes.c_void_p),
+    ]
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _lib():
+    lib = load_library()
+    if not getattr(lib, "_batch_bound", False):
+        lib.kdt_query_batch_device.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_uint64, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p]
+        lib.kdt_query_batch_device.restype = ctypes.c_int
+        lib._batch_bound = True
+    return lib
+
+
+def _check(lib, rc):
+    if rc:
+        raise RuntimeError(lib.kdt_last_error().decode())
+
+
+def _stream_ptr(stream=None) -> ctypes.c_void_p:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+@dataclass
+class DeviceTree:
+    handle: ctypes.c_void_p
+    n: int
+    d: int
+    leafsize: int
+    tune_ms: dict = field(default_factory=dict)
+
+    def free(self):
+        if self.handle is not None and self.handle.value:
+            _lib().kdt_free(self.handle)
+            self.handle = None
+
+
+def build(data, leafsize: int, stream=None) -> DeviceTree:
+    """Build from a (n, d) float64 array (numpy or a CUDA tensor)."""
+    torch = _torch()
+    lib = _lib()
+    t = data if isinstance(data, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(data, np.float64))
+    t = t.to("cuda", dtype=torch.float64).contiguous()
+    h = ctypes.c_void_p()
+    _check(lib, lib.kdt_build_device(ctypes.c_void_p(t.data_ptr()), t.shape[0], t.shape[1], leafsize,
+                                     _stream_ptr(stream), ctypes.byref(h)))
+    return DeviceTree(h, t.shape[0], t.shape[1], leafsize)
+
+
+def query_device(tree: DeviceTree, q, k: int, out_d=None, out_i=None, flags: int = KDT_QUERY_ASSUME_FINITE,
+                 stream=None, p: float = 2.0):
+    torch = _torch()
+    lib = _lib()
+    nq = q.shape[0]
+    if out_d is None:
+        out_d = torch.empty((nq, k), dtype=torch.float64, device=q.device)
+        out_i = torch.empty((nq, k), dtype=torch.int64, device=q.device)
+    _check(lib, lib.kdt_query_device(tree.handle, ctypes.c_void_p(q.data_ptr()), nq, tree.d, k, p, float("inf"), 0.0,
+                                     flags, ctypes.c_void_p(out_d.data_ptr()), ctypes.c_void_p(out_i.data_ptr()),
+                                     _stream_ptr(stream)))
+    return out_d, out_i
+
+
+def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), reps: int = 3) -> DeviceTree:
+    """Build with each leafsize, time `reps` queries of the sample, keep the fastest.
+
+    The leafsize changes the tree, and for 4- and 8-dimensional rows the
+    reference's summation order inside a leaf, so the chosen leafsize is part
+    of the result's definition: the oracle must use the same one.
+    """
+    torch = _torch()
+    q = torch.from_numpy(np.ascontiguousarray(sample_queries, np.float64)).cuda()
+    if not isinstance(data, torch.Tensor):
+        data = torch.from_numpy(np.ascontiguousarray(data, np.float64)).cuda()
+    best, timings = None, {}
+    for ls in candidates:
+        tree = build(data, ls)
+        out_d, out_i = query_device(tree, q, k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            query_device(tree, q, k, out_d, out_i)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        timings[ls] = ms
+        if best is None or ms < timings[best.leafsize]:
+            if best is not None:
+                best.free()
+            best = tree
+        else:
+            tree.free()
+    best.tune_ms = timings
+    return best
+
+
+class Batch:
+    """A fixed set of (tree, queries, k) items with preallocated outputs."""
+
+    def __init__(self, items):
+        torch = _torch()
+        self.items = []
+        self.arr = (BatchItem * len(items))()
+        for j, (tree, q, k) in enumerate(items):
+            out_d = torch.empty((q.shape[0], k), dtype=torch.float64, device=q.device)
+            out_i = torch.empty((q.shape[0], k), dtype=torch.int64, device=q.device)
+            self.items.append((tree, q, k, out_d, out_i))
+            self.arr[j] = BatchItem(tree.handle.value, q.data_ptr(), q.shape[0], tree.d, k, out_d.data_ptr(),
+                                    out_i.data_ptr())
+        self.graph = None
+
+    @property
+    def n_queries(self) -> int:
+        return sum(q.shape[0] for _, q, _, _, _ in self.items)
+
+    def run(self, mode: str = "persistent", stream=None, p: float = 2.0, extra_flags: int = 0):
+        lib = _lib()
+        if mode == "graph":
+            if self.graph is None:
+                self._capture(p, extra_flags)
+            self.graph.replay()
+            return
+        flags = extra_flags | (KDT_BATCH_PER_TREE_LAUNCHES if mode == "per_tree" else 0)
+        _check(lib, lib.kdt_query_batch_device(self.arr, len(self.items), p, float("inf"), 0.0, flags,
+                                               _stream_ptr(stream)))
+
+    def _capture(self, p, extra_flags):
+        torch = _torch()
+        lib = _lib()
+        flags = extra_flags | KDT_BATCH_PER_TREE_LAUNCHES | KDT_BATCH_NO_ORDER
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm the kernels outside capture
+            _check(lib, lib.kdt_query_batch_device(self.arr, len(self.items), p, float("inf"), 0.0, flags,
+                                                   _stream_ptr(s)))
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            _check(lib, lib.kdt_query_batch_device(self.arr, len(self.items), p, float("inf"), 0.0, flags,
+                                                   _stream_ptr(None)))
+        self.graph = g
+
+    def results(self, j: int):
+        _, _, _, out_d, out_i = self.items[j]
+        return out_d, out_i

@@ -74,3 +74,33 @@ def c3_inputs(n: int = C3["n"], nq: int = C3["nq"]):
     data = _mixture(C3["data_seed"], cent.size, n, cent, C3["sigma"])
     queries = _mixture(C3["query_seed"], 0, nq, cent, C3["sigma"])
     return data, queries
+
+
+# Batching configuration: 256 independent trees, n log-uniform in [1e3, 1e6],
+# d uniform from {2, 3, 8, 16}, float32 uniform points (promoted to float64),
+# per-tree query counts log-uniform in [1, 1e5], k from {1, 8, 32}.
+C4 = dict(n_trees=256, seed=4, dims=(2, 3, 8, 16), ks=(1, 8, 32))
+
+
+def uniform_f32(seed: int, start: int, count: int) -> np.ndarray:
+    """float32 uniforms (x >> 40) * 2^-24, promoted to float64."""
+    return ((splitmix64(seed, start, count) >> np.uint64(40)).astype(np.float64) * (1.0 / (1 << 24)))
+
+
+def c4_specs(n_trees: int = C4["n_trees"], scale: float = 1.0):
+    """(n, d, n_queries, k) per tree; `scale` < 1 shrinks n and n_queries for tests."""
+    u = uniform(C4["seed"], 0, 4 * n_trees).reshape(n_trees, 4)
+    specs = []
+    for t in range(n_trees):
+        n = max(2, int(round(10 ** (3 + 3 * u[t, 0]) * scale)))
+        d = C4["dims"][min(3, int(u[t, 1] * 4))]
+        q = max(1, int(round(10 ** (5 * u[t, 2]) * scale)))
+        k = C4["ks"][min(2, int(u[t, 3] * 3))]
+        specs.append((n, d, q, k))
+    return specs
+
+
+def c4_tree_inputs(t: int, n: int, d: int, q: int):
+    data = uniform_f32(10_000 + t, 0, n * d).reshape(n, d)
+    queries = uniform_f32(20_000 + t, 0, q * d).reshape(q, d)
+    return data, queries

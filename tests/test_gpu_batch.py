@@ -1,0 +1,74 @@
+"""Multi-tree batches: one persistent launch, per-tree launches and a replayed
+HIP graph all give, for every item, the oracle's answer for that tree (built
+with the autotuned leafsize)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def forest():
+    import torch
+
+    from kdtree import forest as F
+    from kdtree.workloads import c4_specs, c4_tree_inputs
+
+    specs = c4_specs(n_trees=24, scale=0.03)
+    items, host = [], []
+    for t, (n, d, q, k) in enumerate(specs):
+        data, queries = c4_tree_inputs(t, n, d, q)
+        print(f"tree {t}: n={n} d={d} q={q} k={k}", flush=True)
+        tree = F.autotune_build(data, queries[:512], k, reps=1)
+        items.append((tree, torch.from_numpy(queries).cuda(), k))
+        host.append((data, queries, k, tree.leafsize))
+    batch = F.Batch(items)
+    yield batch, host
+    for tree, _, _ in items:
+        tree.free()
+
+
+def check_all(batch, host):
+    for j, (data, queries, k, ls) in enumerate(host):
+        od, oi = OracleTree(data, ls).query(queries, k=k)
+        gd, gi = batch.results(j)
+        np.testing.assert_array_equal(gi.cpu().numpy(), oi, err_msg=f"item {j}")
+        assert gd.cpu().numpy().tobytes() == od.tobytes(), f"item {j} distances"
+
+
+@pytest.mark.parametrize("mode", ["persistent", "per_tree"])
+def test_batch_modes_match_oracle(forest, mode):
+    import torch
+
+    batch, host = forest
+    batch.run(mode)
+    torch.cuda.synchronize()
+    check_all(batch, host)
+
+
+def test_graph_replays_match_oracle(forest):
+    import torch
+
+    batch, host = forest
+    for replay in range(3):
+        for _, _, _, od, oi in batch.items:  # eager work between replays
+            od.fill_(-1.0)
+            oi.fill_(-1)
+        torch.cuda.synchronize()
+        batch.run("graph")
+        torch.cuda.synchronize()
+        check_all(batch, host)
+
+
+def test_batch_fp32_screen_matches(forest):
+    import torch
+
+    batch, host = forest
+    batch.run("persistent", extra_flags=0x1)
+    torch.cuda.synchronize()
+    check_all(batch, host)

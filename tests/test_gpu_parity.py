@@ -170,3 +170,31 @@ def test_large_query_batch():
     data = np.random.default_rng(90).uniform(size=(200_000, 3))
     queries = np.random.default_rng(91).uniform(size=(20_000, 3))
     _query_case(data, queries, k=16)
+
+
+@pytest.mark.parametrize("dims", [2, 3, 8, 16])
+@pytest.mark.parametrize("k", [1, 8, 10, 16, 32])
+def test_benchmark_shapes_match_oracle(dims, k):
+    # The (ndim, k) pairs of the benchmark configurations.
+    data = np.random.default_rng(100 + dims).normal(size=(6000, dims))
+    queries = np.random.default_rng(200 + k).normal(size=(300, dims))
+    _query_case(data, queries, k=k, leafsize=16)
+
+
+def test_eps_box_order_and_cutoff_together():
+    t = np.random.default_rng(110).uniform(0, 10, size=8000)
+    data = np.stack([t, np.sin(t), 0.01 * np.cos(t)], 1)
+    queries = np.random.default_rng(111).uniform(-1, 11, size=(400, 3))
+    _query_case(data, queries, k=8, leafsize=8, eps=0.3)
+    _query_case(data, queries, k=16, leafsize=8, max_distance=0.4)
+    g = np.stack(np.meshgrid(np.arange(10.0), np.arange(10.0), np.arange(10.0)), -1).reshape(-1, 3)
+    _query_case(np.concatenate([g, g[::3]]), g[::7] + 0.5, k=10, leafsize=4)
+
+
+@pytest.mark.parametrize("order", ["1", "0"])
+def test_query_ordering_is_invisible(monkeypatch, order):
+    # Batches >= 8192 queries run in leaf order; results must not change.
+    monkeypatch.setenv("KDT_NO_QUERY_ORDER", order)
+    data = np.random.default_rng(120).uniform(size=(50_000, 3))
+    queries = np.random.default_rng(121).uniform(size=(20_000, 3))
+    _query_case(data, queries, k=16)

@@ -59,55 +59,76 @@ __global__ __launch_bounds__(256) void k_batch_key(const BItem* __restrict__ ite
   }
 }
 
-__global__ void k_zero(unsigned long long* p) { *p = 0; }
+__global__ void k_zero(unsigned long long* p, uint32_t n) {
+  if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
 
-template <int G, int M, bool SCREEN>
+// One width class of the batch: queries [first, first + count) of the batch
+// order (or of the global numbering when `order` is null).  Groups take
+// queries from `counter` and run them through the phased descent; a group
+// switches to the query's item (tree and outputs) with each new query.
+template <int DT, int G, int M, bool SCREEN>
 __global__ __launch_bounds__(256) void knn_batch_kernel(const BItem* __restrict__ items, uint32_t n_items,
-                                                        uint64_t total, const uint32_t* __restrict__ order,
-                                                        unsigned long long* __restrict__ counter) {
+                                                        const uint32_t* __restrict__ order, uint64_t first,
+                                                        uint64_t count, unsigned long long* __restrict__ counter) {
   constexpr int GPB = 256 / G;
   __shared__ double s_sa[GPB][STK];
   __shared__ double s_sb[GPB][STK];
   __shared__ uint32_t s_sn[GPB][STK];
   __shared__ uint32_t s_sm[GPB][STK];
-  __shared__ double s_ax[GPB][DMAX];
-  __shared__ double s_q[GPB][DMAX];
+  __shared__ double s_ax[GPB][DT];
+  __shared__ double s_q[GPB][DT];
   const uint32_t grp = threadIdx.x / G;
   const uint32_t lane = threadIdx.x & (G - 1);
   RegList<G> best;
-  for (;;) {
+  KArgs cur = items[0].a;  // batch-wide scalars (cutoff, eps, p) from the start
+  auto next = [&](uint64_t& qi) -> bool {
     unsigned long long mine = 0;
     if (lane == 0) mine = atomicAdd(counter, 1ull);
     const uint64_t gq = (uint64_t)__shfl((long long)mine, 0, G);
-    if (gq >= total) break;
-    const uint64_t g = order ? (uint64_t)order[gq] : gq;
+    if (gq >= count) return false;
+    const uint64_t g = order ? (uint64_t)order[first + gq] : first + gq;
     const uint32_t it = find_item(items, n_items, g);
-    const KArgs a = items[it].a;
-    const uint64_t qi = g - items[it].q_begin;
-    switch (a.D) {
-#define KDT_BATCH_WIDTH(DW)                                                                        \
-  case DW:                                                                                         \
-    knn_one<DW, G, M, SCREEN, false>(a, qi, best, s_ax[grp], s_q[grp], s_sa[grp], s_sb[grp],      \
-                                     s_sn[grp], s_sm[grp]);                                        \
-    break;
-      KDT_BATCH_WIDTH(1)
-      KDT_BATCH_WIDTH(2)
-      KDT_BATCH_WIDTH(3)
-      KDT_BATCH_WIDTH(4)
-      KDT_BATCH_WIDTH(8)
-      KDT_BATCH_WIDTH(16)
-#undef KDT_BATCH_WIDTH
-    }
-  }
+    cur = items[it].a;
+    qi = g - items[it].q_begin;
+    return true;
+  };
+  knn_ww<DT, G, M, SCREEN>(cur, best, s_ax[grp], s_q[grp], s_sa[grp], s_sb[grp], s_sn[grp], s_sm[grp], next);
 }
 
-template <int G, int M>
-void launch_batch(const BItem* items, uint32_t n, uint64_t total, const uint32_t* order,
-                  unsigned long long* counter, bool screen, uint32_t blocks, hipStream_t s) {
-  if (screen)
-    knn_batch_kernel<G, M, true><<<blocks, 256, 0, s>>>(items, n, total, order, counter);
-  else
-    knn_batch_kernel<G, M, false><<<blocks, 256, 0, s>>>(items, n, total, order, counter);
+template <int DT, int M>
+void launch_class(int G, bool screen, const BItem* items, uint32_t n, const uint32_t* order, uint64_t first,
+                  uint64_t count, unsigned long long* counter, hipStream_t s) {
+  const uint32_t blocks = 256u * 8u;  // resident grid: ~8 blocks per CU
+#define KDT_CLASS(GG)                                                                                  \
+  if (G == GG) {                                                                                       \
+    if constexpr (M == M_L2) {                                                                         \
+      if (screen) {                                                                                    \
+        knn_batch_kernel<DT, GG, M, true><<<blocks, 256, 0, s>>>(items, n, order, first, count, counter); \
+        return;                                                                                        \
+      }                                                                                                \
+    }                                                                                                  \
+    knn_batch_kernel<DT, GG, M, false><<<blocks, 256, 0, s>>>(items, n, order, first, count, counter); \
+    return;                                                                                            \
+  }
+  KDT_CLASS(8)
+  KDT_CLASS(16)
+  KDT_CLASS(32)
+  KDT_CLASS(64)
+#undef KDT_CLASS
+}
+
+template <int M>
+void launch_class_width(uint64_t d, int G, bool screen, const BItem* items, uint32_t n, const uint32_t* order,
+                        uint64_t first, uint64_t count, unsigned long long* counter, hipStream_t s) {
+  switch (d) {
+    case 1: launch_class<1, M>(G, screen, items, n, order, first, count, counter, s); break;
+    case 2: launch_class<2, M>(G, screen, items, n, order, first, count, counter, s); break;
+    case 3: launch_class<3, M>(G, screen, items, n, order, first, count, counter, s); break;
+    case 4: launch_class<4, M>(G, screen, items, n, order, first, count, counter, s); break;
+    case 8: launch_class<8, M>(G, screen, items, n, order, first, count, counter, s); break;
+    case 16: launch_class<16, M>(G, screen, items, n, order, first, count, counter, s); break;
+  }
 }
 
 }  // namespace
@@ -137,11 +158,13 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   if (rc) return batch_fail(rc, msg);
   base.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   const bool per_tree = (flags & KDT_BATCH_PER_TREE_LAUNCHES) != 0;
+  const bool no_order = (flags & KDT_BATCH_NO_ORDER) != 0;
 
-  std::vector<BItem> h;
-  h.reserve(n_items);
-  uint64_t total = 0, kmax = 0;
-  bool packable = base.metric != M_LP && n_items < (1u << 20);
+  // Validate; split items into width classes that run in shared launches and
+  // the rest (general p, other widths, k > 64), which run one launch each.
+  static const uint64_t widths[] = {1, 2, 3, 4, 8, 16};
+  std::vector<uint64_t> cls_items[6];
+  std::vector<uint64_t> single;
   for (uint64_t i = 0; i < n_items; ++i) {
     const kdt_batch_item& it = items[i];
     if (!it.tree) return batch_fail(KDT_ERR_ARGUMENT, "null tree in batch item " + std::to_string(i));
@@ -152,43 +175,55 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     if (it.ndim != t.d)
       return batch_fail(KDT_ERR_DIMENSION_MISMATCH, "dimension mismatch: expected " + std::to_string(t.d) +
                                                         ", got " + std::to_string(it.ndim));
-    const uint64_t d = t.d;
-    packable &= (d == 1 || d == 2 || d == 3 || d == 4 || d == 8 || d == 16) && it.k <= 64;
-    kmax = std::max<uint64_t>(kmax, it.k);
-    BItem b{};
+    int c = -1;
+    for (int w = 0; w < 6; ++w)
+      if (t.d == widths[w]) c = w;
+    if (per_tree || base.metric == M_LP || c < 0 || it.k > 64) single.push_back(i);
+    else cls_items[c].push_back(i);
+  }
+
+  for (uint64_t i : single) {
+    const kdt_batch_item& it = items[i];
     QueryParams qp = base;
     qp.k = it.k;
     qp.nq = it.n_queries;
-    build_kargs(t, qp, it.queries, it.out_dist, it.out_idx, t.err, b.a);
-    b.q_begin = total;
-    total += it.n_queries;
-    h.push_back(b);
+    qp.allow_order = !no_order;
+    rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, it.tree->t.err, s, msg);
+    if (rc) return batch_fail(rc, msg);
   }
-  if (h.empty()) return KDT_OK;
 
-  if (per_tree || !packable) {
-    for (uint64_t i = 0; i < n_items; ++i) {
+  // Shared launches: items numbered class by class, so each class is one
+  // contiguous range of the batch's query numbers (and of the sorted order).
+  std::vector<BItem> h;
+  uint64_t first[6] = {0}, count[6] = {0}, kmax[6] = {0};
+  uint64_t total = 0;
+  for (int c = 0; c < 6; ++c) {
+    first[c] = total;
+    for (uint64_t i : cls_items[c]) {
       const kdt_batch_item& it = items[i];
-      if (it.n_queries == 0) continue;
+      BItem b{};
       QueryParams qp = base;
       qp.k = it.k;
       qp.nq = it.n_queries;
-      qp.allow_order = !(flags & KDT_BATCH_NO_ORDER);
-      rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, it.tree->t.err, s, msg);
-      if (rc) return batch_fail(rc, msg);
+      build_kargs(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, it.tree->t.err, b.a);
+      b.q_begin = total;
+      total += it.n_queries;
+      kmax[c] = std::max<uint64_t>(kmax[c], it.k);
+      h.push_back(b);
     }
-    return KDT_OK;
+    count[c] = total - first[c];
   }
+  if (h.empty()) return KDT_OK;
+  if (total > 0xFFFFFFFFull) return batch_fail(KDT_ERR_ARGUMENT, "batch exceeds 2^32 queries");
 
   BItem* d_items = nullptr;
-  unsigned long long* counter = nullptr;
+  unsigned long long* counters = nullptr;
   uint32_t* order = nullptr;
   hipError_t e = hipMallocAsync((void**)&d_items, h.size() * sizeof(BItem), s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&counter, sizeof(unsigned long long), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&counters, 6 * sizeof(unsigned long long), s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_items, h.data(), h.size() * sizeof(BItem), hipMemcpyHostToDevice, s);
-  const bool ordered = !(flags & KDT_BATCH_NO_ORDER) && total <= 0xFFFFFFFFull && total >= 8192;
-  if (e == hipSuccess && ordered) {
-    // One sort over the batch: key = (item, leaf of the near path).
+  if (e == hipSuccess && !no_order && total >= 8192) {
+    // One sort over the batch: key = (item, leaf reached by the near path).
     uint32_t leaf_bits = 1;
     uint64_t max_nodes = 1;
     for (const BItem& b : h) max_nodes = std::max<uint64_t>(max_nodes, b.a.n_nodes);
@@ -216,27 +251,28 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     if (tmp) (void)hipFreeAsync(tmp, s);
   }
   if (e == hipSuccess) {
-    k_zero<<<1, 1, 0, s>>>(counter);
-    const int G = kmax <= 16 ? 16 : kmax <= 32 ? 32 : 64;
-    const uint64_t want = (total + (256 / G) - 1) / (256 / G);
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>(want, 256ull * 16);
-    const uint32_t n = (uint32_t)h.size();
+    k_zero<<<1, 64, 0, s>>>(counters, 6);
     const bool screen = base.fp32_screen && base.metric == M_L2;
-    auto go = [&](auto g_tag) {
-      constexpr int GG = decltype(g_tag)::value;
+    const uint32_t n = (uint32_t)h.size();
+    for (int c = 0; c < 6; ++c) {
+      if (!count[c]) continue;
+      const int G = kmax[c] <= 16 ? 16 : kmax[c] <= 32 ? 32 : 64;  // G = 8 measured no faster
       switch (base.metric) {
-        case M_L1: launch_batch<GG, M_L1>(d_items, n, total, order, counter, false, blocks, s); break;
-        case M_L2: launch_batch<GG, M_L2>(d_items, n, total, order, counter, screen, blocks, s); break;
-        default: launch_batch<GG, M_LINF>(d_items, n, total, order, counter, false, blocks, s); break;
+        case M_L1:
+          launch_class_width<M_L1>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, s);
+          break;
+        case M_L2:
+          launch_class_width<M_L2>(widths[c], G, screen, d_items, n, order, first[c], count[c], counters + c, s);
+          break;
+        default:
+          launch_class_width<M_LINF>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, s);
+          break;
       }
-    };
-    if (G == 16) go(std::integral_constant<int, 16>{});
-    else if (G == 32) go(std::integral_constant<int, 32>{});
-    else go(std::integral_constant<int, 64>{});
+    }
     e = hipGetLastError();
   }
   if (order) (void)hipFreeAsync(order, s);
-  if (counter) (void)hipFreeAsync(counter, s);
+  if (counters) (void)hipFreeAsync(counters, s);
   if (d_items) (void)hipFreeAsync(d_items, s);
   if (e != hipSuccess) return batch_fail(KDT_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
   return KDT_OK;

@@ -32,7 +32,7 @@ int group_width(uint64_t k, uint64_t leafsize) {
   int g = k <= 16 ? 16 : k <= 32 ? 32 : 64;
   if (const char* env = std::getenv("KDT_GROUP_WIDTH")) {
     const int want = std::atoi(env);
-    if ((want == 16 || want == 32 || want == 64) && (uint64_t)want >= k) g = want;
+    if ((want == 8 || want == 16 || want == 32 || want == 64) && (uint64_t)want >= k) g = want;
   }
   return g;
 }

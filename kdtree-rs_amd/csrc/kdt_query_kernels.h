@@ -39,6 +39,8 @@
 // in LDS (LdsList, any k that fits).
 #pragma once
 
+#include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "kdt_internal.h"
@@ -147,6 +149,39 @@ __device__ __forceinline__ double box_dist_d(const double* q, const double* __re
       if (off > 0.0) dist += pow(off, p);
     }
     return dist;
+  }
+}
+
+// box_dist by a group of G lanes.  For L1/L2 rows of 8 or 16 the lanes own
+// the dimensions: lane j forms the reference's lane-fold acc_j (one chunk, or
+// two chunks added in order for d = 16) and three xor-shuffles reproduce the
+// pairwise hsum ((a0+a4)+(a2+a6)) + ((a1+a5)+(a3+a7)) bit for bit (IEEE
+// addition commutes).  Other widths and metrics fall back to box_dist_d.
+// Must be called by all G lanes together.
+template <int M, int DT, int G>
+__device__ __forceinline__ double box_dist_grp(const double* q, const double* s_q, const double* __restrict__ lo,
+                                               uint32_t Drt, double p, uint32_t lane) {
+  if constexpr ((M == M_L1 || M == M_L2) && (DT == 8 || DT == 16) && G >= 8) {
+    const double* hi = lo + DT;
+    double v = 0.0;
+    if (lane < 8) {
+      const double qa = s_q[lane];
+      const double o1 = vmax_d(vmax_d(lo[lane] - qa, qa - hi[lane]), 0.0);
+      v = M == M_L2 ? o1 * o1 : fabs(o1);
+      if constexpr (DT == 16) {
+        const double qb = s_q[lane + 8];
+        const double o2 = vmax_d(vmax_d(lo[lane + 8] - qb, qb - hi[lane + 8]), 0.0);
+        v = v + (M == M_L2 ? o2 * o2 : fabs(o2));
+      }
+    }
+    v = v + __shfl_xor(v, 4, G);
+    v = v + __shfl_xor(v, 2, G);
+    v = v + __shfl_xor(v, 1, G);
+    return __shfl(v, 0, G);
+  } else {
+    (void)s_q;
+    (void)lane;
+    return box_dist_d<M, DT>(q, lo, Drt, p);
   }
 }
 
@@ -369,7 +404,7 @@ __device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best,
 
   // CellBound::start / seed, src/query.rs:170-187
   double cell = 0.0;
-  if (box_dist_d<M, DT>(q, a.boxes, D, a.p) != 0.0) {
+  if (box_dist_grp<M, DT, G>(q, s_q, a.boxes, D, a.p, lane) != 0.0) {
     for (uint32_t c = 0; c < D; ++c) {
       const double ax = reduce_d<M>(axis_offset_d(q[c], a.boxes[c], a.boxes[D + c]), a.p);
       axes[c] = ax;
@@ -403,8 +438,8 @@ __device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best,
       const double far_dist = replace_axis_d<M>(cell, old, far_axis);
       if (nd.b & OBB_BIT) {
         // descend_by_box, src/query.rs:298-327
-        const double nb = box_dist_d<M, DT>(q, a.boxes + (size_t)near * 2 * D, D, a.p);
-        const double fb = box_dist_d<M, DT>(q, a.boxes + (size_t)far * 2 * D, D, a.p);
+        const double nb = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)near * 2 * D, D, a.p, lane);
+        const double fb = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)far * 2 * D, D, a.p, lane);
         if (nb <= fb) {
           s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_BOXFAR | (dim << 2);
           ++sp;
@@ -433,34 +468,38 @@ __device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best,
       const uint32_t L = (uint32_t)__double_as_longlong(nd.split) - start;
       const double* base = a.pts + (size_t)start * D;
       const uint32_t full = D == 4 ? (L & ~1u) : D == 8 ? (L & ~7u) : L;
-      for (uint32_t c0 = 0; c0 < L; c0 += G) {
-        const uint32_t o = c0 + lane;
-        double dist = INFINITY;
-        bool hit = false;
-        if (o < L) {
-          bool cand = true;
-          if constexpr (SCREEN) {
-            if (screen_on)
-              cand = point_dist32<DL>(q32, a.pts32 + (size_t)start * D, L, o) <= t32;
-          }
-          if (cand) {
-            dist = point_dist_d<M, DT>(q, base + o, L, D, o < full, a.p);
-            hit = dist <= bound;
-          }
+      // Two chunks of G points per iteration: both chunks' loads are in
+      // flight together; hits are offered chunk by chunk in offset order.
+      const float* base32 = a.pts32 + (size_t)start * D;
+      auto score = [&](uint32_t o, double& dist) -> bool {
+        if (o >= L) return false;
+        if constexpr (SCREEN) {
+          if (screen_on && !(point_dist32<DL>(q32, base32, L, o) <= t32)) return false;
         }
-        uint64_t m = (__ballot(hit) >> gshift) & gmask;
-        if (m) {
-          const uint32_t myidx = hit ? a.idx[start + o] : 0u;
-          do {
-            const int f = __ffsll((unsigned long long)m) - 1;
-            const double df = __shfl(dist, f, G);
-            const uint32_t xf = __shfl(myidx, f, G);
-            bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
-            m &= m - 1;
-            m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
-          } while (m);
-          if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
-        }
+        dist = point_dist_d<M, DT>(q, base + o, L, D, o < full, a.p);
+        return dist <= bound;
+      };
+      auto offer = [&](bool hit, double dist, uint32_t o) {
+        uint64_t m = (__ballot(hit && dist <= bound) >> gshift) & gmask;
+        if (!m) return;
+        const uint32_t myidx = hit ? a.idx[start + o] : 0u;
+        do {
+          const int f = __ffsll((unsigned long long)m) - 1;
+          const double df = __shfl(dist, f, G);
+          const uint32_t xf = __shfl(myidx, f, G);
+          bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
+          m &= m - 1;
+          m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
+        } while (m);
+        if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
+      };
+      for (uint32_t c0 = 0; c0 < L; c0 += 2 * G) {
+        const uint32_t o0 = c0 + lane, o1 = c0 + G + lane;
+        double d0 = INFINITY, d1 = INFINITY;
+        const bool h0 = score(o0, d0);
+        const bool h1 = score(o1, d1);
+        offer(h0, d0, o0);
+        offer(h1, d1, o1);
       }
     }
     // Unwind to the next deferred visit.
@@ -472,13 +511,13 @@ __device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best,
       const uint32_t kind = meta & 3u, edim = meta >> 2;
       if (kind == KIND_RESTORE) { axes[edim] = ea; continue; }
       if (kind == KIND_BOXNEAR) {
-        if (box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound) {
+        if (box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound) {
           node = en; cell = ea; active = true; break;
         }
         continue;
       }
       bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
-      if (ok) ok = box_dist_d<M, DT>(q, a.boxes + (size_t)en * 2 * D, D, a.p) * eps <= bound;
+      if (ok) ok = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound;
       if (ok) {
         s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
         ++sp;
@@ -493,6 +532,219 @@ __device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best,
   best.template write<M>(qi, lane, a);
 }
 
+// Wave-synchronous form of the same descent for the groups of one wavefront.
+// Each group runs its own query through exactly the sequence of knn_one, but
+// the wavefront advances all of its groups phase by phase:
+//   1. descend: every group still descending visits one inner node per
+//      iteration, until each has reached a leaf or must unwind;
+//   2. scan: all groups at a leaf scan it together;
+//   3. unwind: every group pops one stack entry per iteration until it has a
+//      node to descend into or an empty stack (query done);
+// and a group whose query is done writes it and starts its next query at once.
+// Grouping the phases keeps one code path active at a time: with groups left
+// free to interleave, every iteration would issue the inner-node, leaf and
+// unwind code of all groups under divergence masks.  `next(qi)` yields the
+// group's next query row (false when there is none).
+template <int DT, int G, int M, bool SCREEN, typename List, typename Next>
+__device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes, double* s_q, double* s_sa,
+                                       double* s_sb, uint32_t* s_sn, uint32_t* s_sm, Next next) {
+  constexpr int DL = DT > 0 ? DT : 1;
+  constexpr int ST_DESCEND = 0, ST_SCAN = 1, ST_POP = 2, ST_DONE = 3;
+  const uint32_t lane = threadIdx.x & (G - 1);
+  const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
+  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+  const uint32_t D = DT > 0 ? DT : a.D;
+  const double eps = a.eps_factor;
+
+  uint64_t qi = 0;
+  const double* qg = a.q;
+  double qr[DL];
+  float q32[DL];
+  const double* q = qr;
+  double bound = a.cutoff, cell = 0.0, screen_e = 0.0;
+  float t32 = INFINITY;
+  bool screen_on = false;
+  uint32_t node = 0;
+  int sp = 0, state = ST_DONE;
+  DevNode nd{};
+
+  auto start_query = [&]() {
+    qg = a.q + qi * D;
+    double qmax = 0.0;
+    if constexpr (DT > 0) {
+#pragma unroll
+      for (int c = 0; c < DT; ++c) {
+        qr[c] = qg[c];
+        s_q[c] = qr[c];
+        axes[c] = 0.0;
+        q32[c] = __double2float_rn(qr[c]);
+        qmax = fmax(qmax, fabs(qr[c]));
+      }
+    } else {
+      q = qg;
+      for (uint32_t c = 0; c < D; ++c) axes[c] = 0.0;
+    }
+    best.reset();
+    bound = a.cutoff;  // BestK::bound with an empty list
+    if constexpr (SCREEN) {
+      screen_on = qmax <= 1e18 && a.max_abs <= 1e18;
+      screen_e = a.screen_e * fmax(qmax, a.max_abs) + 1e-43;
+      t32 = screen_threshold(bound, screen_e, a);
+    }
+    // CellBound::start / seed, src/query.rs:170-187
+    cell = 0.0;
+    if (box_dist_grp<M, DT, G>(q, s_q, a.boxes, D, a.p, lane) != 0.0) {
+      for (uint32_t c = 0; c < D; ++c) {
+        const double ax = reduce_d<M>(axis_offset_d(q[c], a.boxes[c], a.boxes[D + c]), a.p);
+        axes[c] = ax;
+        cell = fold_d<M>(cell, ax);
+      }
+    }
+    sp = 0;
+    node = 0;
+    state = cell * eps <= bound ? ST_DESCEND : ST_DONE;  // Descent::run, src/query.rs:227-229
+  };
+
+  bool alive = next(qi);
+  if (alive) start_query();
+  while (__ballot(alive)) {
+    // 1. descend (src/query.rs:248-289, :296-327)
+    for (;;) {
+      if (alive && state == ST_DESCEND) {
+        nd = a.nodes[node];
+        const bool leaf = (nd.b & LEAF_BIT) != 0;
+        // Preorder ids grow downwards and leaves lie inside the point range;
+        // anything else ends the query instead of walking off the arrays.
+        if (leaf ? ((uint64_t)__double_as_longlong(nd.split) > a.n_points ||
+                    (uint64_t)__double_as_longlong(nd.split) < nd.a)
+                 : (nd.a <= node || nd.a >= a.n_nodes || (nd.b & DIM_MASK) >= D)) {
+          if (lane == 0) atomicOr(a.err, 2);
+          state = ST_DONE;
+          sp = 0;
+        } else if (leaf) {
+          state = ST_SCAN;
+        } else {
+          const uint32_t dim = nd.b & DIM_MASK;
+          const double qd = DT > 0 ? s_q[dim] : qg[dim];
+          const double diff = qd - nd.split;
+          const bool goleft = diff <= 0.0;
+          const uint32_t near = goleft ? node + 1 : nd.a;
+          const uint32_t far = goleft ? nd.a : node + 1;
+          const double far_axis = reduce_d<M>(fabs(diff), a.p);
+          const double old = axes[dim];
+          const double far_dist = replace_axis_d<M>(cell, old, far_axis);
+          if (nd.b & OBB_BIT) {
+            const double nb = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)near * 2 * D, D, a.p, lane);
+            const double fb = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)far * 2 * D, D, a.p, lane);
+            if (nb <= fb) {
+              s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_BOXFAR | (dim << 2);
+              ++sp;
+              if (nb * eps <= bound) node = near;
+              else state = ST_POP;
+            } else {
+              s_sa[sp] = cell; s_sb[sp] = 0.0; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;
+              ++sp;
+              if (fb * eps <= bound) {  // enter_far, src/query.rs:329-341
+                s_sa[sp] = old; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
+                ++sp;
+                axes[dim] = far_axis;
+                node = far;
+                cell = far_dist;
+              } else {
+                state = ST_POP;
+              }
+            }
+          } else {
+            s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_FAR | (dim << 2);
+            ++sp;
+            node = near;
+          }
+        }
+      }
+      if (!__ballot(alive && state == ST_DESCEND)) break;
+    }
+    // 2. scan_leaf, src/query.rs:343-353; leaf scans, src/kernel.rs:359-563
+    if (alive && state == ST_SCAN) {
+      const uint32_t start = nd.a;
+      const uint32_t L = (uint32_t)__double_as_longlong(nd.split) - start;
+      const double* base = a.pts + (size_t)start * D;
+      const uint32_t full = D == 4 ? (L & ~1u) : D == 8 ? (L & ~7u) : L;
+      const float* base32 = a.pts32 + (size_t)start * D;
+      auto score = [&](uint32_t o, double& dist) -> bool {
+        if (o >= L) return false;
+        if constexpr (SCREEN) {
+          if (screen_on && !(point_dist32<DL>(q32, base32, L, o) <= t32)) return false;
+        }
+        dist = point_dist_d<M, DT>(q, base + o, L, D, o < full, a.p);
+        return dist <= bound;
+      };
+      auto offer = [&](bool hit, double dist, uint32_t o) {
+        uint64_t m = (__ballot(hit && dist <= bound) >> gshift) & gmask;
+        if (!m) return;
+        const uint32_t myidx = hit ? a.idx[start + o] : 0u;
+        do {
+          const int f = __ffsll((unsigned long long)m) - 1;
+          const double df = __shfl(dist, f, G);
+          const uint32_t xf = __shfl(myidx, f, G);
+          bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
+          m &= m - 1;
+          m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
+        } while (m);
+        if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
+      };
+      for (uint32_t c0 = 0; c0 < L; c0 += 2 * G) {
+        const uint32_t o0 = c0 + lane, o1 = c0 + G + lane;
+        double d0 = INFINITY, d1 = INFINITY;
+        const bool h0 = score(o0, d0);
+        const bool h1 = score(o1, d1);
+        offer(h0, d0, o0);
+        offer(h1, d1, o1);
+      }
+      state = ST_POP;
+    }
+    // 3. unwind to the next deferred visit
+    for (;;) {
+      if (alive && state == ST_POP) {
+        if (sp == 0) {
+          state = ST_DONE;
+        } else {
+          --sp;
+          const uint32_t meta = s_sm[sp], en = s_sn[sp];
+          const double ea = s_sa[sp], eb = s_sb[sp];
+          const uint32_t kind = meta & 3u, edim = meta >> 2;
+          if (kind == KIND_RESTORE) {
+            axes[edim] = ea;
+          } else if (kind == KIND_BOXNEAR) {
+            if (box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound) {
+              node = en;
+              cell = ea;
+              state = ST_DESCEND;
+            }
+          } else {
+            bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
+            if (ok) ok = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound;
+            if (ok) {
+              s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
+              ++sp;
+              axes[edim] = ea;
+              node = en;
+              cell = eb;
+              state = ST_DESCEND;
+            }
+          }
+        }
+      }
+      if (!__ballot(alive && state == ST_POP)) break;
+    }
+    // 4. finished queries: results out, next query in
+    if (alive && state == ST_DONE) {
+      best.template write<M>(qi, lane, a);
+      alive = next(qi);
+      if (alive) start_query();
+    }
+  }
+}
+
 template <int DT, int G, int M, bool SCREEN, bool LDSLIST>
 __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s_q, double* s_sa,
                                          double* s_sb, uint32_t* s_sn, uint32_t* s_sm,
@@ -500,14 +752,19 @@ __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s
   constexpr int GPB = 256 / G;
   const uint32_t grp = threadIdx.x / G;
   const uint32_t gpb = LDSLIST ? 1u : (uint32_t)GPB;
-  const uint64_t gid = (uint64_t)blockIdx.x * gpb + grp;
   const uint64_t ngroups = (uint64_t)gridDim.x * gpb;
   typename std::conditional<LDSLIST, LdsList, RegList<G>>::type best;
   if constexpr (LDSLIST) { best.ld = l_d; best.li = l_i; }
-  for (uint64_t qs = gid; qs < a.nq; qs += ngroups) {
-    const uint64_t qi = a.order ? (uint64_t)a.order[qs] : qs;
-    knn_one<DT, G, M, SCREEN, LDSLIST>(a, qi, best, axes, s_q, s_sa, s_sb, s_sn, s_sm);
-  }
+  uint64_t qs = (uint64_t)blockIdx.x * gpb + grp;
+  bool first = true;
+  auto next = [&](uint64_t& qi) -> bool {
+    if (!first) qs += ngroups;
+    first = false;
+    if (qs >= a.nq) return false;
+    qi = a.order ? (uint64_t)a.order[qs] : qs;
+    return true;
+  };
+  knn_ww<DT, G, M, SCREEN>(a, best, axes, s_q, s_sa, s_sb, s_sn, s_sm, next);
 }
 
 template <int DT, int G, int M, bool SCREEN>
@@ -549,8 +806,12 @@ int launch_knn_lds(const KArgs& a, int metric, hipStream_t s);
 template <int DT, int G, int M, bool SCREEN>
 inline void launch_reg(const KArgs& a, hipStream_t s) {
   constexpr int GPB = 256 / G;
+  // One group per query (measured faster than a resident grid that hands
+  // each group several queries); KDT_QUERY_BLOCKS caps the grid for tuning.
+  uint64_t cap = 1u << 30;
+  if (const char* env = std::getenv("KDT_QUERY_BLOCKS")) cap = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
   uint64_t blocks = (a.nq + GPB - 1) / GPB;
-  if (blocks > (1u << 20)) blocks = 1u << 20;
+  if (blocks > cap) blocks = cap;
   knn_reg_kernel<DT, G, M, SCREEN><<<(uint32_t)blocks, 256, 0, s>>>(a);
 }
 
@@ -558,6 +819,7 @@ inline void launch_reg(const KArgs& a, hipStream_t s) {
 // (src/build.rs:60); other widths run the runtime-ndim kernel.
 template <int G, int M, bool SCREEN>
 inline int launch_reg_width(const KArgs& a, hipStream_t s) {
+  static_assert(G == 8 || G == 16 || G == 32 || G == 64, "group width");
   switch (a.D) {
     case 1: launch_reg<1, G, M, SCREEN>(a, s); return 0;
     case 2: launch_reg<2, G, M, SCREEN>(a, s); return 0;

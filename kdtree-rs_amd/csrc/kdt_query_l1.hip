@@ -5,6 +5,7 @@ namespace kdt {
 
 int launch_knn_l1(const KArgs& a, int G, hipStream_t s) {
   switch (G) {
+    case 8: return launch_reg_width<8, M_L1, false>(a, s);
     case 16: return launch_reg_width<16, M_L1, false>(a, s);
     case 32: return launch_reg_width<32, M_L1, false>(a, s);
     case 64: return launch_reg_width<64, M_L1, false>(a, s);

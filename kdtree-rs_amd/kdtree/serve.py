@@ -1,0 +1,169 @@
+"""Continuous-batching k-NN server over resident trees (one GPU).
+
+No reference counterpart: the reference is a library call per query matrix.
+Here trees stay resident in HBM and requests (tree, query rows, k) arrive over
+time.  The dispatcher loop:
+
+  * admits every request whose arrival time has passed into the pending queue;
+  * retires launches whose completion event has fired (results are then in
+    pinned host memory) and frees their share of the HBM admission budget;
+  * while a stream is free and the budget allows, takes pending requests in
+    arrival order up to the per-launch query cap and dispatches them as ONE
+    multi-tree launch (kdt_query_batch_device) on that stream: queries go
+    host->device, the launch runs, results go device->host, then an event.
+
+Requests that arrive while earlier launches run are admitted at the next
+dispatch and run concurrently on another stream, so work from different
+requests overlaps on the device.  Every request's rows are computed by the
+same per-query descent whatever else is in the launch, so a result is the
+same alone and in any mix.  Latency = host-visible completion - arrival.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import forest as F
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds from the start of the run
+    tree: int
+    queries: np.ndarray  # (m, d) float64
+    k: int
+    done: float = -1.0
+    out_d: np.ndarray | None = None
+    out_i: np.ndarray | None = None
+
+
+@dataclass
+class _Launch:
+    stream: object
+    event: object
+    reqs: list
+    host_d: list
+    host_i: list
+    dev: list = field(default_factory=list)
+    nbytes: int = 0
+
+
+class Server:
+    def __init__(self, trees, n_streams: int = 4, hbm_budget_bytes: float = 200e9,
+                 max_queries_per_launch: int = 1 << 18, p: float = 2.0, flags: int = 0):
+        import torch
+
+        self.torch = torch
+        self.trees = trees
+        self.streams = [torch.cuda.Stream() for _ in range(n_streams)]  # default priority
+        self.budget = hbm_budget_bytes
+        self.cap = max_queries_per_launch
+        self.p = p
+        self.flags = flags
+        self.lib = F._lib()
+        self.max_inflight = 0
+        self.launches = 0
+
+    def _bytes(self, r: Request) -> int:
+        m, d = r.queries.shape
+        return m * d * 8 + m * r.k * 16
+
+    def _dispatch(self, reqs, stream) -> _Launch:
+        torch = self.torch
+        items = (F.BatchItem * len(reqs))()
+        dev, host_d, host_i = [], [], []
+        with torch.cuda.stream(stream):
+            for j, r in enumerate(reqs):
+                m, d = r.queries.shape
+                qh = torch.from_numpy(r.queries).pin_memory()
+                qd = qh.to("cuda", non_blocking=True)
+                od = torch.empty((m, r.k), dtype=torch.float64, device="cuda")
+                oi = torch.empty((m, r.k), dtype=torch.int64, device="cuda")
+                t = self.trees[r.tree]
+                items[j] = F.BatchItem(t.handle.value, qd.data_ptr(), m, t.d, r.k, od.data_ptr(), oi.data_ptr())
+                dev.append((qh, qd, od, oi))
+            rc = self.lib.kdt_query_batch_device(items, len(reqs), self.p, float("inf"), 0.0, self.flags,
+                                                 ctypes.c_void_p(stream.cuda_stream))
+            if rc:
+                raise RuntimeError(self.lib.kdt_last_error().decode())
+            for (_, _, od, oi) in dev:
+                hd = torch.empty(od.shape, dtype=torch.float64, pin_memory=True)
+                hi = torch.empty(oi.shape, dtype=torch.int64, pin_memory=True)
+                hd.copy_(od, non_blocking=True)
+                hi.copy_(oi, non_blocking=True)
+                host_d.append(hd)
+                host_i.append(hi)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return _Launch(stream, ev, list(reqs), host_d, host_i, dev, sum(self._bytes(r) for r in reqs))
+
+    def run(self, requests, time_limit: float = 600.0):
+        """Serve `requests` (sorted by arrival); returns them with results and completion times."""
+        pending: deque = deque()
+        upcoming = deque(sorted(requests, key=lambda r: r.arrival))
+        inflight: list[_Launch] = []
+        free = list(self.streams)
+        used = 0
+        t0 = time.perf_counter()
+        while upcoming or pending or inflight:
+            now = time.perf_counter() - t0
+            if now > time_limit:
+                raise TimeoutError("serving run exceeded its time limit")
+            while upcoming and upcoming[0].arrival <= now:
+                pending.append(upcoming.popleft())
+            for L in list(inflight):
+                if L.event.query():
+                    tdone = time.perf_counter() - t0
+                    for r, hd, hi in zip(L.reqs, L.host_d, L.host_i):
+                        r.out_d, r.out_i, r.done = hd.numpy(), hi.numpy(), tdone
+                    inflight.remove(L)
+                    free.append(L.stream)
+                    used -= L.nbytes
+            while pending and free:
+                take, nq, nb = [], 0, 0
+                while pending and (not take or nq + pending[0].queries.shape[0] <= self.cap):
+                    b = self._bytes(pending[0])
+                    if used + nb + b > self.budget and (take or inflight):
+                        break
+                    r = pending.popleft()
+                    take.append(r)
+                    nq += r.queries.shape[0]
+                    nb += b
+                if not take:
+                    break
+                L = self._dispatch(take, free.pop())
+                inflight.append(L)
+                used += L.nbytes
+                self.launches += 1
+                self.max_inflight = max(self.max_inflight, len(inflight))
+            if not pending or not free:
+                time.sleep(0.0002)
+        return requests
+
+
+def latency_stats(requests):
+    lat = np.array([r.done - r.arrival for r in requests]) * 1e3
+    nq = sum(r.queries.shape[0] for r in requests)
+    span = max(r.done for r in requests) - min(r.arrival for r in requests)
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "queries_per_s": nq / span, "requests": len(requests),
+            "queries": int(nq), "span_s": span}
+
+
+def poisson_requests(n_requests: int, rate: float, tree_dims, k: int, max_batch: int, seed: int,
+                     make_queries):
+    """Arrivals at `rate` per second; tree uniform; batch size log-uniform in [1, max_batch]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    out = []
+    for i in range(n_requests):
+        tree = int(rng.integers(0, len(tree_dims)))
+        m = max(1, int(round(10 ** (np.log10(max_batch) * rng.uniform()))))
+        out.append(Request(i, float(t[i]), tree, make_queries(i, m, tree_dims[tree]), k))
+    return out

@@ -1,0 +1,45 @@
+"""The serving loop: requests that overlap on the device get exactly the result
+they get alone, and the oracle's."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+
+def test_served_results_equal_alone_and_oracle():
+    import torch
+
+    from kdtree import forest as F
+    from kdtree.serve import Server, latency_stats, poisson_requests
+
+    dims = [16, 64, 16, 64]
+    rng = np.random.default_rng(7)
+    datas = [rng.standard_normal((int(n), d)).astype(np.float32).astype(np.float64)
+             for n, d in zip([3000, 1500, 5000, 2500], dims)]
+    trees = [F.build(x, 32) for x in datas]
+
+    def make_q(i, m, d):
+        return np.random.default_rng(1000 + i).standard_normal((m, d)).astype(np.float32).astype(np.float64)
+
+    reqs = poisson_requests(40, rate=2000.0, tree_dims=dims, k=10, max_batch=400, seed=3, make_queries=make_q)
+    server = Server(trees, n_streams=3, max_queries_per_launch=600)
+    server.run(reqs, time_limit=120)
+    stats = latency_stats(reqs)
+    print(stats, "max_inflight", server.max_inflight, "launches", server.launches, flush=True)
+    assert server.launches < len(reqs)  # requests were coalesced
+    for r in reqs:
+        q = torch.from_numpy(r.queries).cuda()
+        ad, ai = F.query_device(trees[r.tree], q, r.k, flags=0)
+        np.testing.assert_array_equal(ai.cpu().numpy(), r.out_i)
+        assert ad.cpu().numpy().tobytes() == r.out_d.tobytes()
+    for r in reqs[:6]:
+        od, oi = OracleTree(datas[r.tree], 32).query(r.queries, k=r.k)
+        np.testing.assert_array_equal(oi, r.out_i)
+        assert od.tobytes() == r.out_d.tobytes()
+    for t in trees:
+        t.free()

@@ -50,7 +50,11 @@ enum kdt_error {
 
 /* Query flags. */
 #define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 leaf screen with a proven error margin,
-                                      FP64 recheck: identical results, fewer bytes */
+                                      FP64 recheck: identical results, fewer bytes.
+                                      For 32 <= ndim <= 64, p = 2, eps = 0, k <= 16:
+                                      matrix-core screen over every point, FP64
+                                      recheck and a visit certificate; queries the
+                                      certificate does not cover run the traversal */
 #define KDT_QUERY_ASSUME_FINITE 0x2u /* kdt_query_device only: the caller has already
                                         checked that every query coordinate is finite
                                         (e.g. by an earlier call on the same buffer), so
@@ -58,6 +62,11 @@ enum kdt_error {
                                         synchronising the stream.  Batches of 8192 or
                                         more queries still take stream-ordered scratch
                                         (hipMallocAsync) for the leaf-order sort. */
+
+#define KDT_QUERY_SCREEN_ONLY 0x4u /* diagnostic, with KDT_QUERY_FP32_SCREEN on the
+                                      matrix-core path: skip the traversal of the
+                                      queries the certificate does not cover, whose
+                                      output rows are then left as they were */
 
 /* Batch flags (kdt_query_batch_device). */
 #define KDT_BATCH_PER_TREE_LAUNCHES 0x100u /* one launch per item instead of one persistent launch */

@@ -67,13 +67,14 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   kdt::QueryParams qp;
   qp.k = k;
   qp.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
+  qp.screen_only = (flags & KDT_QUERY_SCREEN_ONLY) != 0;
   std::string smsg;
   const int scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
   if (q_on_device && out_on_device && (flags & KDT_QUERY_ASSUME_FINITE)) {
     if (scalar_rc) return fail(scalar_rc, smsg);
     qp.nq = nq;
     std::string msg;
-    const int rc = kdt::launch_query(t, qp, q, out_d, out_i, t.err, s, msg);
+    const int rc = kdt::launch_query(t, qp, q, out_d, out_i, s, msg);
     return rc ? fail(rc, msg) : KDT_OK;
   }
 
@@ -114,10 +115,14 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
     }
     double* od = out_on_device ? out_d + b * k : dd.p;
     int64_t* oi = out_on_device ? out_i + b * k : di.p;
+    if (qp.screen_only && !out_on_device) {  // rows the screen skips keep the caller's contents
+      API_HIP(hipMemcpyAsync(dd.p, out_d + b * k, m * k * 8, hipMemcpyHostToDevice, s));
+      API_HIP(hipMemcpyAsync(di.p, out_i + b * k, m * k * 8, hipMemcpyHostToDevice, s));
+    }
     kdt::QueryParams qpt = qp;
     qpt.nq = m;
     std::string msg;
-    const int rc = kdt::launch_query(t, qpt, dqp, od, oi, flag.p, s, msg);
+    const int rc = kdt::launch_query(t, qpt, dqp, od, oi, s, msg);
     if (rc) return fail(rc, msg);
     if (!out_on_device) {
       API_HIP(hipMemcpyAsync(out_d + b * k, dd.p, m * k * 8, hipMemcpyDeviceToHost, s));
@@ -127,8 +132,7 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   int h = 0;
   API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
   API_HIP(hipStreamSynchronize(s));
-  if (h & 1) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
-  if (h & 2) return fail(KDT_ERR_DEVICE, "internal error: malformed tree met during the query");
+  if (h) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
   return KDT_OK;
 }
 
@@ -263,7 +267,7 @@ int kdt_export(const kdt_tree* tree, double* data, uint32_t* indices, kdt_node_r
 void kdt_free(kdt_tree* tree) {
   if (!tree) return;
   kdt::Tree& t = tree->t;
-  void* ptrs[] = {t.nodes, t.boxes, t.pts, t.pts32, t.idx, t.rows, t.err};
+  void* ptrs[] = {t.nodes, t.boxes, t.pts, t.pts32, t.idx, t.rows, t.rows32, t.pnorm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete tree;

@@ -7,7 +7,7 @@
 // by (item, leaf reached by the near path) so that queries sharing a wavefront
 // share a tree and nodes; the groups of a persistent grid then take queries
 // from an atomic counter until the batch is drained and run the same per-query
-// descent as the single-tree kernels (knn_one), so every result is identical
+// descent as the single-tree kernels (knn_ww), so every result is identical
 // to a kdt_query_device call on the same item.
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -21,8 +21,6 @@
 
 namespace kdt {
 namespace {
-
-constexpr int DMAX = 16;  // widest compiled row width
 
 struct BItem {
   KArgs a;
@@ -51,9 +49,7 @@ __global__ __launch_bounds__(256) void k_batch_key(const BItem* __restrict__ ite
       const DevNode nd = a.nodes[node];
       if (nd.b & LEAF_BIT) break;
       const uint32_t dim = nd.b & DIM_MASK;
-      const uint32_t next = q[dim] - nd.split <= 0.0 ? node + 1 : nd.a;
-      if (next <= node || next >= a.n_nodes || dim >= a.D) break;
-      node = next;
+      node = q[dim] - nd.split <= 0.0 ? node + 1 : nd.a;
     }
     key[g] = ((uint64_t)it << leaf_bits) | node;
   }
@@ -134,9 +130,7 @@ void launch_class_width(uint64_t d, int G, bool screen, const BItem* items, uint
 }  // namespace
 
 int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out, int64_t* d_idx,
-                int* d_err, KArgs& a);
-int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out, int64_t* d_idx,
-                 int* d_err, hipStream_t s, std::string& msg);
+                KArgs& a);
 
 }  // namespace kdt
 
@@ -188,7 +182,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     qp.k = it.k;
     qp.nq = it.n_queries;
     qp.allow_order = !no_order;
-    rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, it.tree->t.err, s, msg);
+    rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, s, msg);
     if (rc) return batch_fail(rc, msg);
   }
 
@@ -205,7 +199,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
       QueryParams qp = base;
       qp.k = it.k;
       qp.nq = it.n_queries;
-      build_kargs(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, it.tree->t.err, b.a);
+      build_kargs(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, b.a);
       b.q_begin = total;
       total += it.n_queries;
       kmax[c] = std::max<uint64_t>(kmax[c], it.k);

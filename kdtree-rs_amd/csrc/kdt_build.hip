@@ -30,6 +30,7 @@
 // one node; counts and offsets per chunk are scanned per node in between.
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <vector>
 
@@ -651,8 +652,55 @@ __global__ __launch_bounds__(64) void k_leaf_buckets(const double* __restrict__ 
     const size_t c = e / L, o = e - c * L;
     const double v = rows[base + (size_t)o * D + c];
     pts[base + e] = v;
-    pts32[base + e] = (float)v;
+    if (pts32) pts32[base + e] = (float)v;
   }
+}
+
+// One check of every node after the build, so that the query kernels can
+// read nodes unchecked: an inner node's right child lies after it and inside
+// the node array and its split dim is a dimension; a leaf's rows [start, end)
+// lie inside the points.  Sets bit 8 of *err on a violation.
+__global__ void k_validate_nodes(const DevNode* __restrict__ nodes, uint64_t n_nodes, uint64_t n_points,
+                                 uint32_t D, int* err) {
+  bool bad = false;
+  for (uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; id < n_nodes;
+       id += (uint64_t)gridDim.x * blockDim.x) {
+    const DevNode nd = nodes[id];
+    if (nd.b & LEAF_BIT) {
+      const uint64_t end = (uint64_t)__double_as_longlong(nd.split);
+      bad |= end > n_points || end < nd.a;
+    } else {
+      bad |= nd.a <= id || nd.a >= n_nodes || (nd.b & DIM_MASK) >= D;
+      bad |= id + 1 >= n_nodes;
+    }
+  }
+  if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(err, 8);
+}
+
+// Matrix-core screen rows: tree-order rows rounded to f32, zero-padded to
+// `stride` columns and n_pad rows; pnorm = |row32|^2 (+inf on padding rows);
+// *pmax_bits = max over rows of |row32| rounded up, as f32 bits.
+__global__ void k_rows32(const double* __restrict__ rows, uint64_t n, uint32_t D, uint32_t stride,
+                         uint64_t n_pad, float* __restrict__ rows32, float* __restrict__ pnorm,
+                         uint32_t* pmax_bits) {
+  float mx = 0.0f;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_pad;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (uint32_t c = 0; c < stride; ++c) {
+      const float v = (r < n && c < D) ? (float)rows[r * D + c] : 0.0f;
+      rows32[r * stride + c] = v;
+      s += (double)v * (double)v;
+    }
+    pnorm[r] = r < n ? (float)s : INFINITY;
+    if (r < n) {
+      const double nr = sqrt(s);
+      float f = (float)nr;
+      if ((double)f < nr) f = nextafterf(f, INFINITY);
+      mx = fmaxf(mx, f);
+    }
+  }
+  atomicMax(pmax_bits, __float_as_uint(mx));  // non-negative floats order as integers
 }
 
 __global__ void k_original_order(const double* __restrict__ rows, const uint32_t* __restrict__ idx,
@@ -764,7 +812,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<LvlNode> d_lvl;
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
-  DevBuf<float> pts32;
+  DevBuf<float> pts32, rows32, pnorm;
   DevBuf<uint32_t> idx, tidx, alist, blist, cnt, off, hist, node_len, d_leaves;
   DevBuf<uint64_t> ukey;
   DevBuf<Sel> sel;
@@ -777,7 +825,14 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   KDT_HIP_CHECK(part.alloc(max_chunks * 2 * D));
   KDT_HIP_CHECK(boxes.alloc(t.n_nodes * 2 * D));
   KDT_HIP_CHECK(pts.alloc(n * D));
-  KDT_HIP_CHECK(pts32.alloc(n * D));
+  if (D <= 16) KDT_HIP_CHECK(pts32.alloc(n * D));  // the FP32 screen of the compiled widths
+  const bool wide = D >= 32 && D <= 64;
+  const uint32_t stride32 = D <= 32 ? 32 : 64;
+  const uint64_t n_pad = (n + 31) / 32 * 32;
+  if (wide) {
+    KDT_HIP_CHECK(rows32.alloc(n_pad * stride32));
+    KDT_HIP_CHECK(pnorm.alloc(n_pad));
+  }
   KDT_HIP_CHECK(idx.alloc(n));
   KDT_HIP_CHECK(tidx.alloc(n));
   KDT_HIP_CHECK(alist.alloc(n));
@@ -790,14 +845,14 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   KDT_HIP_CHECK(ukey.alloc(n));
   KDT_HIP_CHECK(sel.alloc(max_slots));
   KDT_HIP_CHECK(nodes.alloc(t.n_nodes));
-  KDT_HIP_CHECK(err.alloc(2));  // [0] error bits, [1] largest fix-up need of a level
+  KDT_HIP_CHECK(err.alloc(3));  // [0] error bits, [1] largest fix-up need of a level, [2] pmax bits
 
   KDT_HIP_CHECK(hipMemcpyAsync(d_lvl.p, all_nodes.data(), tot_nodes * sizeof(LvlNode), hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(d_chunks.p, all_chunks.data(), tot_chunks * sizeof(Chunk), hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(d_leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(rows.p, d_data, n * D * 8, hipMemcpyDeviceToDevice, s));
   KDT_HIP_CHECK(hipMemsetAsync(hist.p, 0, max_slots * 256 * 4, s));
-  KDT_HIP_CHECK(hipMemsetAsync(err.p, 0, 8, s));
+  KDT_HIP_CHECK(hipMemsetAsync(err.p, 0, 12, s));
 
   // Finiteness of the input (the root bbox check, src/build.rs:51-57).
   launch_check_finite(rows.p, n * D, err.p, s);
@@ -863,14 +918,30 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   k_order_by_box<<<(uint32_t)((t.n_nodes + 255) / 256), 256, 0, s>>>(nodes.p, t.n_nodes, boxes.p, Du,
                                                                     node_len.p);
   k_leaf_buckets<<<(uint32_t)(leaves.size() / 2), 64, 0, s>>>(rows.p, Du, d_leaves.p, pts.p, pts32.p);
+  if (wide) {
+    k_rows32<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
+        rows.p, n, Du, stride32, n_pad, rows32.p, pnorm.p, reinterpret_cast<uint32_t*>(err.p) + 2);
+  }
+  k_validate_nodes<<<(uint32_t)std::min<uint64_t>((t.n_nodes + 255) / 256, 4096), 256, 0, s>>>(
+      nodes.p, t.n_nodes, n, Du, err.p);
   KDT_HIP_CHECK(hipGetLastError());
 
   std::vector<double> rb(2 * D);
   KDT_HIP_CHECK(hipMemcpyAsync(rb.data(), boxes.p, 2 * D * 8, hipMemcpyDeviceToHost, s));
   KDT_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, 4, hipMemcpyDeviceToHost, s));
+  uint32_t h_pmax = 0;
+  KDT_HIP_CHECK(hipMemcpyAsync(&h_pmax, err.p + 2, 4, hipMemcpyDeviceToHost, s));
   KDT_HIP_CHECK(hipStreamSynchronize(s));
+  if (h_err & 8) {
+    msg = "internal error: malformed node after the build";
+    return 100;
+  }
   if (h_err) {
     msg = "internal error: partition counts disagree";
+    return 100;
+  }
+  if (t.depth > 32) {  // the query stack holds two entries per inner level
+    msg = "internal error: tree deeper than 32 levels";
     return 100;
   }
   double mx = 0.0;
@@ -882,9 +953,13 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   t.pts32 = pts32.release();
   t.idx = idx.release();
   t.rows = rows.release();
-  KDT_HIP_CHECK(hipMalloc(&t.err, sizeof(int)));
-  KDT_HIP_CHECK(hipMemsetAsync(t.err, 0, sizeof(int), s));
-  KDT_HIP_CHECK(hipStreamSynchronize(s));
+  if (wide) {
+    float pm;
+    std::memcpy(&pm, &h_pmax, 4);
+    t.pmax = (double)pm * (1.0 + 1e-6);
+    t.rows32 = rows32.release();
+    t.pnorm = pnorm.release();
+  }
   return 0;
 }
 

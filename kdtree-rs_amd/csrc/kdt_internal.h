@@ -9,6 +9,11 @@
 //   idx    u32[n_points]               original row of tree position
 //   rows   f64[n_points][ndim]         rows in tree order (row-major), the
 //                                      reference's Tree::data
+//   rows32 f32[n_pad][stride]          32 <= ndim <= 64 only: rows in tree order
+//                                      rounded to f32, zero-padded to stride 32
+//                                      or 64 and to n_pad = 32 * ceil(n / 32)
+//                                      rows, for the matrix-core screen
+//   pnorm  f32[n_pad]                  |row32|^2 (+inf on padding rows)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,7 +50,9 @@ struct Tree {
   uint32_t* idx = nullptr;
   double* rows = nullptr;
   double max_abs = 0.0;  // max |coordinate| over the data (root box)
-  int* err = nullptr;     // device int for kernel error bits of unchecked calls
+  float* rows32 = nullptr;
+  float* pnorm = nullptr;
+  double pmax = 0.0;     // upper bound on max |row32|
 };
 
 // Build on the device from row-major f64 data already in device memory.
@@ -57,6 +64,7 @@ struct QueryParams {
   int metric = M_L2;
   double p = 2.0, cutoff = 0.0, eps_factor = 1.0;
   bool fp32_screen = false;
+  bool screen_only = false;  // diagnostic: no traversal after the matrix-core screen
   bool allow_order = true;  // large batches may run in leaf order
 };
 
@@ -71,9 +79,13 @@ std::string rust_f64(double x);
 void set_last_error(std::string msg);
 
 // Launch the exact k-NN query; d_q [nq x d] f64, outputs [nq x k].
-// *d_err gets bit 2 if the kernel meets a malformed tree.
 int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
-                 int64_t* d_idx, int* d_err, hipStream_t s, std::string& msg);
+                 int64_t* d_idx, hipStream_t s, std::string& msg);
+
+// Matrix-core screen for 32 <= ndim <= 64 (kdt_screen.hip).
+struct KArgs;
+bool screen_applies(const Tree& t, const QueryParams& qp);
+int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg);
 
 // Non-finite check of n doubles; sets *d_flag to 1 if any is NaN or infinite.
 void launch_check_finite(const double* d_x, uint64_t n, int* d_flag, hipStream_t s);

@@ -28,9 +28,7 @@ __global__ __launch_bounds__(256) void k_leaf_key(const DevNode* __restrict__ no
       const DevNode nd = nodes[node];
       if (nd.b & LEAF_BIT) break;
       const uint32_t dim = nd.b & DIM_MASK;
-      const uint32_t next = qi[dim] - nd.split <= 0.0 ? node + 1 : nd.a;
-      if (next <= node || next >= n_nodes || dim >= D) break;  // malformed tree: stop
-      node = next;
+      node = qi[dim] - nd.split <= 0.0 ? node + 1 : nd.a;  // nodes validated by the build
     }
     key[i] = node;
   }

@@ -96,7 +96,7 @@ int query_params(double p, double max_distance, double eps, QueryParams& qp, std
 }
 
 int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out, int64_t* d_idx,
-                int* d_err, KArgs& a) {
+                KArgs& a) {
   a = KArgs{};
   a.nodes = t.nodes;
   a.boxes = t.boxes;
@@ -112,7 +112,6 @@ int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double*
   a.k = (uint32_t)qp.k;
   a.n_points = (uint32_t)t.n;
   a.n_nodes = (uint32_t)t.n_nodes;
-  a.err = d_err;
   a.cutoff = qp.cutoff;
   a.eps_factor = qp.eps_factor;
   a.p = qp.p;
@@ -127,10 +126,11 @@ int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double*
 }
 
 int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
-                 int64_t* d_idx, int* d_err, hipStream_t s, std::string& msg) {
+                 int64_t* d_idx, hipStream_t s, std::string& msg) {
   KArgs a;
-  build_kargs(t, qp, d_q, d_out, d_idx, d_err, a);
+  build_kargs(t, qp, d_q, d_out, d_idx, a);
   if (qp.nq == 0) return 0;
+  if (screen_applies(t, qp)) return launch_screen(t, qp, a, s, msg);
 
   // Large batches run grouped by leaf (see kdt_order.hip).
   uint32_t* order = nullptr;

@@ -47,6 +47,9 @@
 
 namespace kdt {
 
+// Stack entries per group.  The build checks every node (validate_nodes:
+// preorder child ids, split dims, leaf ranges) and that the depth is at most
+// 32, so the traversal reads nodes unchecked and never overflows the stack.
 constexpr int STK = 64;
 constexpr uint32_t KIND_FAR = 0, KIND_BOXFAR = 1, KIND_BOXNEAR = 2, KIND_RESTORE = 3;
 
@@ -59,9 +62,9 @@ struct KArgs {
   const uint32_t* __restrict__ idx;
   const double* __restrict__ q;
   const uint32_t* __restrict__ order;  // processing order of the queries (null: as given)
+  const uint8_t* __restrict__ only;    // non-null: run only the queries qi with only[qi] != 0
   double* __restrict__ out_d;
   int64_t* __restrict__ out_i;
-  int* err;             // set to 2 on a malformed tree (never on a built one)
   uint64_t nq;
   uint32_t D, k, n_points, n_nodes;
   double cutoff, eps_factor, p, inv_p;
@@ -298,6 +301,48 @@ struct RegList {
     }
     return bound;
   }
+  // Several offers at once: lane o offers (d, x) when `valid`.  The list after
+  // BestK::consider of the offers in any order is the lexicographic top-k of
+  // (list + offers), so the offers are bitonic-sorted across the group, the
+  // lower half of (list, reversed offers) is kept (half-cleaner) and merged
+  // into order.  Returns the new bound.
+  static constexpr bool kBulk = true;
+  __device__ __forceinline__ double insert_many(double d, uint32_t x, bool valid, uint32_t nvalid, double bound,
+                                                uint32_t lane, const KArgs& a) {
+    auto lless = [](double d1, uint32_t i1, double d2, uint32_t i2) {
+      return d1 < d2 || (d1 == d2 && i1 < i2);
+    };
+    double bd = valid ? d : INFINITY;
+    uint32_t bi = valid ? x : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t size = 2; size <= G; size <<= 1) {
+#pragma unroll
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        const double pd = __shfl_xor(bd, stride, G);
+        const uint32_t pi = __shfl_xor(bi, stride, G);
+        const bool up = (lane & size) == 0 || size == G;
+        const bool lower = (lane & stride) == 0;
+        const bool take = (lower == up) ? lless(pd, pi, bd, bi) : lless(bd, bi, pd, pi);
+        if (take) { bd = pd; bi = pi; }
+      }
+    }
+    // lane j: min(list[j], offers[G-1-j]) -- the G smallest of both, bitonic
+    const double rd = __shfl(bd, G - 1 - lane, G);
+    const uint32_t ri = __shfl(bi, G - 1 - lane, G);
+    if (lless(rd, ri, kd, ki)) { kd = rd; ki = ri; }
+#pragma unroll
+    for (uint32_t stride = G >> 1; stride > 0; stride >>= 1) {
+      const double pd = __shfl_xor(kd, stride, G);
+      const uint32_t pi = __shfl_xor(ki, stride, G);
+      const bool lower = (lane & stride) == 0;
+      const bool take = lower ? lless(pd, pi, kd, ki) : lless(kd, ki, pd, pi);
+      if (take) { kd = pd; ki = pi; }
+    }
+    cnt = cnt + nvalid < a.k ? cnt + nvalid : a.k;
+    if (lane >= cnt) { kd = INFINITY; ki = 0xFFFFFFFFu; }
+    if (cnt == a.k) bound = fmin(__shfl(kd, a.k - 1, G), a.cutoff);
+    return bound;
+  }
   // write_results, src/query.rs:483-493
   template <int M>
   __device__ __forceinline__ void write(uint64_t qi, uint32_t lane, const KArgs& a) const {
@@ -311,9 +356,14 @@ struct RegList {
 
 // Sorted array in LDS for any k that fits (one group of 64 lanes per block).
 struct LdsList {
+  static constexpr bool kBulk = false;
   double* ld;
   uint32_t* li;
   uint32_t cnt;
+  __device__ __forceinline__ double insert_many(double, uint32_t, bool, uint32_t, double bound, uint32_t,
+                                                const KArgs&) {
+    return bound;
+  }
   __device__ __forceinline__ void reset() { cnt = 0; }
   __device__ __forceinline__ double insert(double d, uint32_t x, double bound, uint32_t lane,
                                            uint32_t, uint64_t, const KArgs& a) {
@@ -360,181 +410,11 @@ struct LdsList {
 // SCREEN: FP32 leaf screen + FP64 recheck (L2, DT > 0 only).  LDSLIST: k list
 // in LDS (G must be 64, one group per block; dynamic LDS holds the list and,
 // for DT == 0, the cell-bound axes).
-// One query: the descent of Descent::run (src/query.rs:223-232) for query row
-// qi, by the calling group of G lanes; writes row qi of the outputs.
-template <int DT, int G, int M, bool SCREEN, bool LDSLIST, typename List>
-__device__ __forceinline__ void knn_one(const KArgs& a, uint64_t qi, List& best, double* axes, double* s_q,
-                                        double* s_sa, double* s_sb, uint32_t* s_sn, uint32_t* s_sm) {
-  constexpr int DL = DT > 0 ? DT : 1;
-  const uint32_t lane = threadIdx.x & (G - 1);
-  const uint32_t gshift = (threadIdx.x & 63) & ~(uint32_t)(G - 1);
-  const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
-  const uint32_t D = DT > 0 ? DT : a.D;
-  const double eps = a.eps_factor;
-  const double* qg = a.q + qi * D;
-  double qr[DL];
-  float q32[DL];
-  const double* q;
-  double qmax = 0.0;
-  if constexpr (DT > 0) {
-#pragma unroll
-    for (int c = 0; c < DT; ++c) {
-      qr[c] = qg[c];
-      s_q[c] = qr[c];
-      axes[c] = 0.0;
-      q32[c] = __double2float_rn(qr[c]);
-      qmax = fmax(qmax, fabs(qr[c]));
-    }
-    q = qr;
-  } else {
-    q = qg;
-    for (uint32_t c = 0; c < D; ++c) axes[c] = 0.0;
-  }
-  (void)q32;
-  best.reset();
-  double bound = a.cutoff;  // BestK::bound with an empty list
-  float t32 = INFINITY;
-  double screen_e = 0.0;
-  bool screen_on = false;
-  if constexpr (SCREEN) {
-    screen_on = qmax <= 1e18 && a.max_abs <= 1e18;
-    screen_e = a.screen_e * fmax(qmax, a.max_abs) + 1e-43;
-    t32 = screen_threshold(bound, screen_e, a);
-  }
-
-  // CellBound::start / seed, src/query.rs:170-187
-  double cell = 0.0;
-  if (box_dist_grp<M, DT, G>(q, s_q, a.boxes, D, a.p, lane) != 0.0) {
-    for (uint32_t c = 0; c < D; ++c) {
-      const double ax = reduce_d<M>(axis_offset_d(q[c], a.boxes[c], a.boxes[D + c]), a.p);
-      axes[c] = ax;
-      cell = fold_d<M>(cell, ax);
-    }
-  }
-  int sp = 0;
-  uint32_t node = 0;
-  bool active = cell * eps <= bound;  // Descent::run, src/query.rs:227-229
-  while (active) {
-    const DevNode nd = a.nodes[node];
-    const bool leaf = (nd.b & LEAF_BIT) != 0;
-    // Preorder ids grow downwards and leaves lie inside the point range;
-    // anything else ends the query instead of walking off the arrays.
-    if (leaf ? ((uint64_t)__double_as_longlong(nd.split) > a.n_points ||
-                (uint64_t)__double_as_longlong(nd.split) < nd.a)
-             : (nd.a <= node || nd.a >= a.n_nodes || (nd.b & DIM_MASK) >= D)) {
-      if (lane == 0) atomicOr(a.err, 2);
-      break;
-    }
-    if (!leaf) {
-      // descend, src/query.rs:255-287
-      const uint32_t dim = nd.b & DIM_MASK;
-      const double qd = DT > 0 ? s_q[dim] : qg[dim];
-      const double diff = qd - nd.split;
-      const bool goleft = diff <= 0.0;
-      const uint32_t near = goleft ? node + 1 : nd.a;
-      const uint32_t far = goleft ? nd.a : node + 1;
-      const double far_axis = reduce_d<M>(fabs(diff), a.p);
-      const double old = axes[dim];
-      const double far_dist = replace_axis_d<M>(cell, old, far_axis);
-      if (nd.b & OBB_BIT) {
-        // descend_by_box, src/query.rs:298-327
-        const double nb = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)near * 2 * D, D, a.p, lane);
-        const double fb = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)far * 2 * D, D, a.p, lane);
-        if (nb <= fb) {
-          s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_BOXFAR | (dim << 2);
-          ++sp;
-          if (nb * eps <= bound) { node = near; continue; }
-        } else {
-          s_sa[sp] = cell; s_sb[sp] = 0.0; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;
-          ++sp;
-          if (fb * eps <= bound) {  // enter_far, src/query.rs:329-341
-            s_sa[sp] = old; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
-            ++sp;
-            axes[dim] = far_axis;
-            node = far;
-            cell = far_dist;
-            continue;
-          }
-        }
-      } else {
-        s_sa[sp] = far_axis; s_sb[sp] = far_dist; s_sn[sp] = far; s_sm[sp] = KIND_FAR | (dim << 2);
-        ++sp;
-        node = near;
-        continue;
-      }
-    } else {
-      // scan_leaf, src/query.rs:343-353; leaf scans, src/kernel.rs:359-563
-      const uint32_t start = nd.a;
-      const uint32_t L = (uint32_t)__double_as_longlong(nd.split) - start;
-      const double* base = a.pts + (size_t)start * D;
-      const uint32_t full = D == 4 ? (L & ~1u) : D == 8 ? (L & ~7u) : L;
-      // Two chunks of G points per iteration: both chunks' loads are in
-      // flight together; hits are offered chunk by chunk in offset order.
-      const float* base32 = a.pts32 + (size_t)start * D;
-      auto score = [&](uint32_t o, double& dist) -> bool {
-        if (o >= L) return false;
-        if constexpr (SCREEN) {
-          if (screen_on && !(point_dist32<DL>(q32, base32, L, o) <= t32)) return false;
-        }
-        dist = point_dist_d<M, DT>(q, base + o, L, D, o < full, a.p);
-        return dist <= bound;
-      };
-      auto offer = [&](bool hit, double dist, uint32_t o) {
-        uint64_t m = (__ballot(hit && dist <= bound) >> gshift) & gmask;
-        if (!m) return;
-        const uint32_t myidx = hit ? a.idx[start + o] : 0u;
-        do {
-          const int f = __ffsll((unsigned long long)m) - 1;
-          const double df = __shfl(dist, f, G);
-          const uint32_t xf = __shfl(myidx, f, G);
-          bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
-          m &= m - 1;
-          m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
-        } while (m);
-        if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
-      };
-      for (uint32_t c0 = 0; c0 < L; c0 += 2 * G) {
-        const uint32_t o0 = c0 + lane, o1 = c0 + G + lane;
-        double d0 = INFINITY, d1 = INFINITY;
-        const bool h0 = score(o0, d0);
-        const bool h1 = score(o1, d1);
-        offer(h0, d0, o0);
-        offer(h1, d1, o1);
-      }
-    }
-    // Unwind to the next deferred visit.
-    active = false;
-    while (sp > 0) {
-      --sp;
-      const uint32_t meta = s_sm[sp], en = s_sn[sp];
-      const double ea = s_sa[sp], eb = s_sb[sp];
-      const uint32_t kind = meta & 3u, edim = meta >> 2;
-      if (kind == KIND_RESTORE) { axes[edim] = ea; continue; }
-      if (kind == KIND_BOXNEAR) {
-        if (box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound) {
-          node = en; cell = ea; active = true; break;
-        }
-        continue;
-      }
-      bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
-      if (ok) ok = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound;
-      if (ok) {
-        s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
-        ++sp;
-        axes[edim] = ea;
-        node = en;
-        cell = eb;
-        active = true;
-        break;
-      }
-    }
-  }
-  best.template write<M>(qi, lane, a);
-}
-
-// Wave-synchronous form of the same descent for the groups of one wavefront.
-// Each group runs its own query through exactly the sequence of knn_one, but
-// the wavefront advances all of its groups phase by phase:
+//
+// The descent of Descent::run (src/query.rs:223-232), wave-synchronous.
+// Each group runs its own query through exactly the reference's sequence of
+// visits, gates and bound updates, but the wavefront advances all of its
+// groups phase by phase:
 //   1. descend: every group still descending visits one inner node per
 //      iteration, until each has reached a leaf or must unwind;
 //   2. scan: all groups at a leaf scan it together;
@@ -613,15 +493,7 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
       if (alive && state == ST_DESCEND) {
         nd = a.nodes[node];
         const bool leaf = (nd.b & LEAF_BIT) != 0;
-        // Preorder ids grow downwards and leaves lie inside the point range;
-        // anything else ends the query instead of walking off the arrays.
-        if (leaf ? ((uint64_t)__double_as_longlong(nd.split) > a.n_points ||
-                    (uint64_t)__double_as_longlong(nd.split) < nd.a)
-                 : (nd.a <= node || nd.a >= a.n_nodes || (nd.b & DIM_MASK) >= D)) {
-          if (lane == 0) atomicOr(a.err, 2);
-          state = ST_DONE;
-          sp = 0;
-        } else if (leaf) {
+        if (leaf) {
           state = ST_SCAN;
         } else {
           const uint32_t dim = nd.b & DIM_MASK;
@@ -682,6 +554,14 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
         uint64_t m = (__ballot(hit && dist <= bound) >> gshift) & gmask;
         if (!m) return;
         const uint32_t myidx = hit ? a.idx[start + o] : 0u;
+        if constexpr (List::kBulk) {
+          const uint32_t nv = (uint32_t)__popcll(m);
+          if (nv >= 4) {
+            bound = best.insert_many(dist, myidx, hit && dist <= bound, nv, bound, lane, a);
+            if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
+            return;
+          }
+        }
         do {
           const int f = __ffsll((unsigned long long)m) - 1;
           const double df = __shfl(dist, f, G);
@@ -758,11 +638,13 @@ __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s
   uint64_t qs = (uint64_t)blockIdx.x * gpb + grp;
   bool first = true;
   auto next = [&](uint64_t& qi) -> bool {
-    if (!first) qs += ngroups;
-    first = false;
-    if (qs >= a.nq) return false;
-    qi = a.order ? (uint64_t)a.order[qs] : qs;
-    return true;
+    for (;;) {
+      if (!first) qs += ngroups;
+      first = false;
+      if (qs >= a.nq) return false;
+      qi = a.order ? (uint64_t)a.order[qs] : qs;
+      if (!a.only || a.only[qi]) return true;
+    }
   };
   knn_ww<DT, G, M, SCREEN>(a, best, axes, s_q, s_sa, s_sb, s_sn, s_sm, next);
 }

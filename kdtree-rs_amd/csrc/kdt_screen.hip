@@ -1,0 +1,478 @@
+// kdt_screen.hip -- exact k-NN for wide rows (32 <= ndim <= 64, p = 2,
+// eps = 0) by an MFMA distance screen, an FP64 recheck and a visit
+// certificate.
+//
+// Replaces, for these widths, the leaf scans of the reference's descent
+// (Streamed plan, src/kernel.rs:296-303) without changing any result.  At
+// such widths the split planes of a median tree cut few of the dimensions,
+// so the descent of src/query.rs:234-341 visits nearly every leaf (all of
+// them for 64-d Gaussian rows): the work is a dense query x point
+// contraction, which is what the matrix cores do.
+//
+// Steps for a query matrix against one tree:
+//   1. screen pass 1 (k_screen<.., 1>): for a tile of 32 queries and a slice
+//      of the points, v_mfma_f32_32x32x2f32 forms q.p in FP32 and each lane
+//      keeps the k smallest  A = |q|^2 + |p|^2 - 2 q.p  of the points it sees;
+//   2. k_screen_tau: the k-th smallest A over all slices gives A_k, and from
+//      it a threshold tau_A such that every point whose FP64 distance could
+//      be among the k best has A <= tau_A (error bound below);
+//   3. screen pass 2 (k_screen<.., 2>): the same products again; points with
+//      A <= tau_A are appended to the query's candidate list;
+//   4. k_screen_recheck: FP64 distances of the candidates in the reference's
+//      summation order (point_dist_d), lexicographic (distance, index) top-k,
+//      then the certificate: for each of the k points, every gate on its
+//      root-to-leaf path (plane bound and box bound of each far child, box
+//      bound of each child of an order_by_box node, the root's seeded cell)
+//      is <= the k-th distance.  The reference's bound never drops below the
+//      k-th distance, so it opens every one of those gates: it visits every
+//      point of this top-k, and its answer (the top-k of what it visits) is
+//      this top-k.  Queries whose candidates overflow, or whose certificate
+//      fails, are flagged;
+//   5. the traversal kernels run the flagged queries (KArgs::only).
+//
+// FP32 error bound (u = 2^-24, q~ and p~ the rows rounded to FP32, P the
+// largest |p~| of the tree, N = |q~| + P):
+//   |A - |q~ - p~|^2| <= E2 = (2 g + 8u) N^2 + D 2^-125,  g = (2D+4)u/(1-(2D+4)u)
+//     (MFMA accumulation of D products, FP32 rounding of |q~|^2, |p~|^2, the
+//      sum and the final fma; flushed subnormal products);
+//   | |q~ - p~| - |q - p| | <= e1 = u (1 + 2u) N + sqrt(D) 2^-148;
+//   the FP64 reduced distance d of the reference satisfies d / s <= |q-p|^2
+//     <= d s with s = 1 + 2 (D + 3) 2^-53.
+// So U(A) = s (sqrt(A + E2) + e1)^2 bounds d from above, and a point with
+// d <= tau has A <= tau_A = (sqrt(s tau) + e1)^2 + E2.  With tau =
+// min(U(A_k), cutoff): at least k points have d <= U(A_k), so the k-th best
+// distance is <= tau and every point of the top-k is a candidate.
+// Rows with a coordinate above 1e18 in magnitude are not screened (flagged).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+#include "kdt_query_kernels.h"
+
+namespace kdt {
+
+int launch_knn_lds(const KArgs& a, int metric, hipStream_t s);
+
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t CAND_CAP = 64;  // candidates per query (one per lane of the recheck)
+constexpr int KP = 16;             // largest k on this path
+constexpr uint32_t MAX_SLICES = 32;
+
+struct SArgs {
+  const float* __restrict__ rows32;  // [n_pad][2 KS], tree order, zero-padded
+  const float* __restrict__ pnorm;   // [n_pad] |p~|^2, +inf on padding rows
+  const double* __restrict__ q;      // [nq][D]
+  float* __restrict__ part;          // pass 1: [nq][2 S][k]
+  const float* __restrict__ tau;     // pass 2: [nq]
+  uint32_t* __restrict__ cnt;        // pass 2: [nq]
+  uint32_t* __restrict__ cand;       // pass 2: [nq][CAND_CAP]
+  uint64_t nq;
+  uint32_t D, k, n, tiles_per_slice, n_tiles, slices;
+};
+
+// One wave: 32 queries (B operand, columns) against 32-row tiles of points
+// (A operand, rows).  Lane l holds query j = l & 31, coordinates
+// [h KS, h KS + KS) with h = l >> 5, in both operands, so instruction s
+// contracts coordinates s and KS + s.  The accumulator holds D[i][j] for
+// point rows i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
+template <int KS, int PASS>
+__global__ __launch_bounds__(64) void k_screen(SArgs a) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t j = lane & 31, h = lane >> 5;
+  const uint64_t qrow = (uint64_t)blockIdx.x * 32 + j;
+  const bool qvalid = qrow < a.nq;
+  float qf[KS];
+  double qn_part = 0.0;
+#pragma unroll
+  for (int s = 0; s < KS; ++s) {
+    const uint32_t c = h * KS + s;
+    const float v = (qvalid && c < a.D) ? (float)a.q[qrow * a.D + c] : 0.0f;
+    qf[s] = v;
+    qn_part += (double)v * (double)v;
+  }
+  const float qn = (float)(qn_part + __shfl_xor(qn_part, 32));
+  const uint32_t t0 = blockIdx.y * a.tiles_per_slice;
+  const uint32_t t1 = min(a.n_tiles, t0 + a.tiles_per_slice);
+
+  float list[KP];  // PASS 1: KP - k sentinels (-inf) then the k smallest A, ascending
+  float thr = INFINITY;
+  if constexpr (PASS == 1) {
+#pragma unroll
+    for (int e = 0; e < KP; ++e) list[e] = (e < KP - (int)a.k) ? -INFINITY : INFINITY;
+  } else {
+    thr = qvalid ? a.tau[qrow] : -INFINITY;
+  }
+
+  constexpr int NV = KS / 4;
+  float4 cur[NV], nxt[NV];
+  float4 pn_cur[4], pn_nxt[4];
+  auto load = [&](uint32_t t, float4* dst, float4* pn) {
+    const float4* src = reinterpret_cast<const float4*>(a.rows32 + ((size_t)t * 32 + j) * (2 * KS) + h * KS);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) dst[v] = src[v];
+    const float4* ps = reinterpret_cast<const float4*>(a.pnorm + (size_t)t * 32 + 4 * h);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) pn[g] = ps[2 * g];
+  };
+  if (t0 < t1) load(t0, cur, pn_cur);
+  for (uint32_t t = t0; t < t1; ++t) {
+    if (t + 1 < t1) load(t + 1, nxt, pn_nxt);
+    floatx16 acc = {};
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].x, qf[4 * v + 0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].y, qf[4 * v + 1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].z, qf[4 * v + 2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].w, qf[4 * v + 3], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float4 pg = pn_cur[r / 4];
+      const float pn = (r % 4 == 0) ? pg.x : (r % 4 == 1) ? pg.y : (r % 4 == 2) ? pg.z : pg.w;
+      const float A = fmaf(-2.0f, acc[r], qn + pn);
+      if constexpr (PASS == 1) {
+        if (A < thr) {  // insert, dropping the largest
+#pragma unroll
+          for (int e = KP - 1; e > 0; --e) list[e] = fminf(list[e], fmaxf(list[e - 1], A));
+          list[0] = fminf(list[0], A);
+          thr = list[KP - 1];
+        }
+      } else {
+        const uint32_t row = t * 32 + 8 * (r / 4) + 4 * h + (r % 4);
+        if (A <= thr && row < a.n) {
+          const uint32_t c = atomicAdd(&a.cnt[qrow], 1u);
+          if (c < CAND_CAP) a.cand[qrow * CAND_CAP + c] = row;
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) pn_cur[g] = pn_nxt[g];
+  }
+  if constexpr (PASS == 1) {
+    if (qvalid) {
+      float* out = a.part + (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * a.k;
+#pragma unroll
+      for (int e = 0; e < KP; ++e)
+        if (e >= KP - (int)a.k) out[e - (KP - (int)a.k)] = list[e];
+    }
+  }
+}
+
+struct TauArgs {
+  const float* __restrict__ part;
+  const double* __restrict__ q;
+  float* __restrict__ tau;
+  uint32_t* __restrict__ cnt;
+  uint8_t* __restrict__ flag;
+  uint64_t nq;
+  uint32_t D, k, lists;
+  double pmax, cutoff, max_abs;
+};
+
+// One wave per query: k rounds of wave-min over the heads of the sorted
+// per-lane lists give A_k; then tau_A from the error bound.  Also clears the
+// query's candidate count and fallback flag.
+__global__ __launch_bounds__(64) void k_screen_tau(TauArgs a) {
+  const uint64_t qi = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const float* lst = a.part + (qi * a.lists + lane) * a.k;
+  uint32_t head = 0;
+  const bool mine = lane < a.lists;
+  float cur = mine ? lst[0] : INFINITY;
+  float ak = INFINITY;
+  for (uint32_t r = 0; r < a.k; ++r) {
+    float m = cur;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+    ak = m;
+    const uint64_t hit = __ballot(mine && cur == m);
+    if (hit && lane == (uint32_t)(__ffsll((unsigned long long)hit) - 1)) {
+      ++head;
+      cur = head < a.k ? lst[head] : INFINITY;
+    }
+  }
+  double qn = 0.0, qmax = 0.0;
+  for (uint32_t c = lane; c < a.D; c += 64) {
+    const double v = a.q[qi * a.D + c];
+    const double f = (double)(float)v;
+    qn += f * f;
+    qmax = fmax(qmax, fabs(v));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    qn += __shfl_xor(qn, o);
+    qmax = fmax(qmax, __shfl_xor(qmax, o));
+  }
+  if (lane == 0) {
+    const double u = 0x1p-24;
+    const double D = (double)a.D;
+    const double m2 = (2.0 * D + 4.0) * u;
+    const double g = m2 / (1.0 - m2);
+    const double N = (sqrt(qn) * (1.0 + 0x1p-50) + a.pmax);
+    const double E2 = ((2.0 * g + 8.0 * u) * N * N + D * 0x1p-125) * (1.0 + 1e-12);
+    const double e1 = (u * (1.0 + 2.0 * u) * N + sqrt(D) * 0x1p-148) * (1.0 + 1e-12);
+    const double sh = 1.0 + 2.0 * (D + 3.0) * 0x1p-53;
+    double tau = a.cutoff;
+    if (ak < INFINITY) {
+      const double r = sqrt(fmax(0.0, (double)ak + E2)) + e1;
+      tau = fmin(tau, sh * r * r * (1.0 + 1e-12));
+    }
+    float t32 = INFINITY;
+    if (tau < INFINITY) {
+      const double r = sqrt(sh * tau) + e1;
+      const double ta = (r * r + E2) * (1.0 + 1e-12);
+      t32 = (float)ta;
+      if ((double)t32 < ta) t32 = nextafterf(t32, INFINITY);
+    }
+    const bool ok = qmax <= 1e18 && a.max_abs <= 1e18;
+    a.tau[qi] = ok ? t32 : -INFINITY;
+    a.flag[qi] = ok ? 0 : 1;
+    a.cnt[qi] = 0;
+  }
+}
+
+struct CheckArgs {
+  const double* __restrict__ q;
+  const double* __restrict__ rows;
+  const uint32_t* __restrict__ idx;
+  const DevNode* __restrict__ nodes;
+  const double* __restrict__ boxes;
+  const uint32_t* __restrict__ cnt;
+  const uint32_t* __restrict__ cand;
+  uint8_t* __restrict__ flag;
+  double* __restrict__ out_d;
+  int64_t* __restrict__ out_i;
+  uint64_t nq;
+  uint32_t D, k, n;
+  double cutoff;
+};
+
+__device__ __forceinline__ bool key_less(double da, uint32_t ia, double db, uint32_t ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// One wave per query: FP64 recheck of the candidates, top-k, certificate.
+// Dynamic LDS: k x D doubles, the cell-bound axes of each certificate walk.
+__global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
+  extern __shared__ double axes_all[];
+  const uint64_t qi = blockIdx.x;
+  const uint32_t lane = threadIdx.x;
+  const uint32_t D = a.D;
+  if (a.flag[qi]) return;
+  const uint32_t c = a.cnt[qi];
+  if (c > CAND_CAP) {
+    if (lane == 0) a.flag[qi] = 1;
+    return;
+  }
+  const double* q = a.q + qi * D;
+  double d = INFINITY;
+  uint32_t id = 0xFFFFFFFFu, pos = 0;
+  if (lane < c) {
+    pos = a.cand[qi * CAND_CAP + lane];
+    const double v = point_dist_d<M_L2, 0>(q, a.rows + (size_t)pos * D, 1, D, true, 2.0);
+    if (v <= a.cutoff) {
+      d = v;
+      id = a.idx[pos];
+    }
+  }
+  // Bitonic sort of the 64 (distance, index, position) keys, ascending.
+  for (uint32_t size = 2; size <= 64; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      const double od = __shfl_xor(d, stride);
+      const uint32_t oid = __shfl_xor(id, stride);
+      const uint32_t opos = __shfl_xor(pos, stride);
+      const bool up = (lane & size) == 0;
+      const bool lower = (lane & stride) == 0;
+      const bool other_less = key_less(od, oid, d, id);
+      const bool take = (lower == up) ? other_less : !other_less && !(od == d && oid == id);
+      if (take) {
+        d = od;
+        id = oid;
+        pos = opos;
+      }
+    }
+  }
+  const uint32_t nvalid = (uint32_t)__popcll(__ballot(id != 0xFFFFFFFFu));
+  const uint32_t nres = min(nvalid, a.k);
+  const double kth = __shfl(d, (int)(a.k - 1));
+  const double dcert = nres == a.k ? kth : a.cutoff;
+
+  bool ok = true;
+  if (lane < nres) {
+    double* axes = axes_all + (size_t)lane * D;
+    double cell = 0.0;
+    const double root = box_dist_d<M_L2, 0>(q, a.boxes, D, 2.0);
+    for (uint32_t j = 0; j < D; ++j) axes[j] = 0.0;
+    if (root != 0.0) {  // CellBound::seed, src/query.rs:170-187
+      for (uint32_t j = 0; j < D; ++j) {
+        const double ax = reduce_d<M_L2>(axis_offset_d(q[j], a.boxes[j], a.boxes[D + j]), 2.0);
+        axes[j] = ax;
+        cell = fold_d<M_L2>(cell, ax);
+      }
+    }
+    ok = cell <= dcert;  // Descent::run, src/query.rs:227-229
+    uint32_t node = 0, start = 0, len = a.n;
+    while (ok) {
+      const DevNode nd = a.nodes[node];
+      if (nd.b & LEAF_BIT) break;
+      const uint32_t dim = nd.b & DIM_MASK;
+      const double diff = q[dim] - nd.split;
+      const bool goleft = diff <= 0.0;
+      const uint32_t mid = len / 2;  // split_point, src/build.rs:161-164
+      const bool to_left = pos < start + mid;
+      const bool is_far = to_left != goleft;
+      const uint32_t child = to_left ? node + 1 : nd.a;
+      const double far_axis = reduce_d<M_L2>(fabs(diff), 2.0);
+      const double far_dist = replace_axis_d<M_L2>(cell, axes[dim], far_axis);
+      if (nd.b & OBB_BIT) {  // descend_by_box, src/query.rs:296-327
+        ok = box_dist_d<M_L2, 0>(q, a.boxes + (size_t)child * 2 * D, D, 2.0) <= dcert;
+      } else if (is_far) {   // descend, src/query.rs:278-286
+        ok = far_dist <= dcert && box_dist_d<M_L2, 0>(q, a.boxes + (size_t)child * 2 * D, D, 2.0) <= dcert;
+      }
+      if (is_far) {          // enter_far, src/query.rs:329-341
+        axes[dim] = far_axis;
+        cell = far_dist;
+      }
+      if (to_left) {
+        len = mid;
+      } else {
+        start += mid;
+        len -= mid;
+      }
+      node = child;
+    }
+  }
+  if (__ballot(!ok)) {
+    if (lane == 0) a.flag[qi] = 1;
+    return;
+  }
+  if (lane < a.k) {  // write_results, src/query.rs:382-400
+    a.out_d[qi * a.k + lane] = lane < nres ? sqrt(d) : INFINITY;
+    a.out_i[qi * a.k + lane] = lane < nres ? (int64_t)id : (int64_t)a.n;
+  }
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count, hipStream_t s) {
+  return hipMallocAsync((void**)p, std::max<size_t>(1, count) * sizeof(T), s) == hipSuccess ? 0 : 1;
+}
+
+}  // namespace
+
+bool screen_applies(const Tree& t, const QueryParams& qp) {
+  return qp.fp32_screen && qp.metric == M_L2 && qp.eps_factor == 1.0 && qp.k <= (uint64_t)KP && t.rows32 &&
+         qp.nq < (1ull << 31) &&
+         t.d >= 32 && t.d <= 64 && std::getenv("KDT_NO_MFMA_SCREEN") == nullptr;
+}
+
+int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg) {
+  const uint64_t nq = qp.nq;
+  const uint32_t k = (uint32_t)qp.k;
+  const uint64_t qtiles = (nq + 31) / 32;
+  const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
+  // Slices of the points per query tile: enough waves to fill the device,
+  // at least 8 tiles per slice.
+  uint64_t slices = std::max<uint64_t>(1, (4096 + qtiles - 1) / qtiles);
+  slices = std::min<uint64_t>({slices, MAX_SLICES, std::max<uint32_t>(1, n_tiles / 8)});
+  const uint32_t per = (uint32_t)((n_tiles + slices - 1) / slices);
+  slices = (n_tiles + per - 1) / per;
+
+  float *part = nullptr, *tau = nullptr;
+  uint32_t *cnt = nullptr, *cand = nullptr;
+  uint8_t* flag = nullptr;
+  int bad = 0;
+  bad |= dev_alloc(&part, nq * 2 * slices * k, s);
+  bad |= dev_alloc(&tau, nq, s);
+  bad |= dev_alloc(&cnt, nq, s);
+  bad |= dev_alloc(&cand, nq * CAND_CAP, s);
+  bad |= dev_alloc(&flag, nq, s);
+  auto release = [&]() {
+    for (void* p : {(void*)part, (void*)tau, (void*)cnt, (void*)cand, (void*)flag})
+      if (p) (void)hipFreeAsync(p, s);
+  };
+  if (bad) {
+    release();
+    msg = "screen scratch allocation failed";
+    return 100;
+  }
+  SArgs sa{};
+  sa.rows32 = t.rows32;
+  sa.pnorm = t.pnorm;
+  sa.q = trav.q;
+  sa.part = part;
+  sa.tau = tau;
+  sa.cnt = cnt;
+  sa.cand = cand;
+  sa.nq = nq;
+  sa.D = (uint32_t)t.d;
+  sa.k = k;
+  sa.n = (uint32_t)t.n;
+  sa.tiles_per_slice = per;
+  sa.n_tiles = n_tiles;
+  sa.slices = (uint32_t)slices;
+  const dim3 grid((uint32_t)qtiles, (uint32_t)slices);
+  const bool narrow = t.d <= 32;
+  if (narrow) k_screen<16, 1><<<grid, 64, 0, s>>>(sa);
+  else k_screen<32, 1><<<grid, 64, 0, s>>>(sa);
+
+  TauArgs ta{};
+  ta.part = part;
+  ta.q = trav.q;
+  ta.tau = tau;
+  ta.cnt = cnt;
+  ta.flag = flag;
+  ta.nq = nq;
+  ta.D = (uint32_t)t.d;
+  ta.k = k;
+  ta.lists = (uint32_t)(2 * slices);
+  ta.pmax = t.pmax;
+  ta.cutoff = qp.cutoff;
+  ta.max_abs = t.max_abs;
+  k_screen_tau<<<(uint32_t)nq, 64, 0, s>>>(ta);
+
+  if (narrow) k_screen<16, 2><<<grid, 64, 0, s>>>(sa);
+  else k_screen<32, 2><<<grid, 64, 0, s>>>(sa);
+
+  CheckArgs ca{};
+  ca.q = trav.q;
+  ca.rows = t.rows;
+  ca.idx = t.idx;
+  ca.nodes = t.nodes;
+  ca.boxes = t.boxes;
+  ca.cnt = cnt;
+  ca.cand = cand;
+  ca.flag = flag;
+  ca.out_d = trav.out_d;
+  ca.out_i = trav.out_i;
+  ca.nq = nq;
+  ca.D = (uint32_t)t.d;
+  ca.k = k;
+  ca.n = (uint32_t)t.n;
+  ca.cutoff = qp.cutoff;
+  k_screen_recheck<<<(uint32_t)nq, 64, (size_t)k * t.d * 8, s>>>(ca);
+
+  // Flagged queries: the exact traversal.
+  KArgs fb = trav;
+  fb.order = nullptr;
+  fb.only = flag;
+  const int rc = qp.screen_only ? 0 : launch_knn_lds(fb, M_L2, s);
+  release();
+  if (rc) {
+    msg = "no fallback kernel";
+    return 101;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    msg = std::string("screen launch: ") + hipGetErrorString(e);
+    return 100;
+  }
+  return 0;
+}
+
+}  // namespace kdt

@@ -1,0 +1,108 @@
+"""Matrix-core screen for 32 <= ndim <= 64 (p = 2, eps = 0, k <= 16).
+
+Every result must equal the oracle's (the reference's descent restated) bit
+for bit, whichever way a query is answered: by the screen + FP64 recheck +
+visit certificate, or by the traversal for the queries the certificate does
+not cover.  The screen-only diagnostic flag skips that traversal, so the rows
+it writes show what the matrix-core path itself answered.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from kdtree import KDTree
+from oracle import OracleTree
+
+pytestmark = pytest.mark.gpu
+
+FP32_SCREEN = 0x1
+SCREEN_ONLY = 0x4
+
+
+def f32_normal(seed: int, n: int, d: int) -> np.ndarray:
+    return np.random.default_rng(seed).standard_normal((n, d)).astype(np.float32).astype(np.float64)
+
+
+def mixture(seed: int, n: int, d: int, sigma: float) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0, 10, size=(16, d))
+    return centres[rng.integers(0, 16, size=n)] + sigma * rng.standard_normal((n, d))
+
+
+# name: (data, queries, k, leafsize, max_distance, (min, max) fraction of the
+# queries the screen answers by itself).  Duplicated rows overflow the
+# candidate list, so every such query must take the traversal.
+CASES = {
+    "normal-f32-20000x64-k10": (lambda: f32_normal(1, 20_000, 64), lambda: f32_normal(2, 1000, 64), 10, 32, None, (0.95, 1.0)),
+    "normal-f32-6000x32-k16": (lambda: f32_normal(3, 6000, 32), lambda: f32_normal(4, 777, 32), 16, 32, None, (0.95, 1.0)),
+    "normal-3001x33-k1": (lambda: np.random.default_rng(5).normal(size=(3001, 33)),
+                          lambda: np.random.default_rng(6).normal(size=(65, 33)), 1, 32, None, (0.95, 1.0)),
+    "normal-777x48-k16-ls8": (lambda: np.random.default_rng(7).normal(size=(777, 48)),
+                              lambda: np.random.default_rng(8).normal(size=(300, 48)), 16, 8, None, (0.95, 1.0)),
+    "uniform-f64-10000x40-k5": (lambda: np.random.default_rng(9).uniform(size=(10_000, 40)),
+                                lambda: np.random.default_rng(10).uniform(size=(500, 40)), 5, 32, None, (0.95, 1.0)),
+    "mixture-8000x64-k10": (lambda: mixture(11, 8000, 64, 0.3), lambda: mixture(12, 400, 64, 0.3), 10, 16, None, (0.0, 1.0)),
+    "duplicates-4000x64-k10": (lambda: np.repeat(f32_normal(13, 40, 64), 100, axis=0),
+                               lambda: f32_normal(14, 100, 64), 10, 32, None, (0.0, 0.0)),
+    "tiny-5x64-k10": (lambda: f32_normal(15, 5, 64), lambda: f32_normal(16, 33, 64), 10, 32, None, (0.95, 1.0)),
+    "cutoff-5000x64-k16": (lambda: f32_normal(17, 5000, 64), lambda: f32_normal(18, 200, 64), 16, 32, 10.0, (0.95, 1.0)),
+    "far-queries-5000x32-k4": (lambda: f32_normal(19, 5000, 32), lambda: 50.0 + f32_normal(20, 100, 32), 4, 32,
+                               None, (0.0, 1.0)),
+}
+
+
+def screen_only(tree: KDTree, q: np.ndarray, k: int, md):
+    out_d = np.full((q.shape[0], k), np.nan)
+    out_i = np.full((q.shape[0], k), -7, dtype=np.int64)
+    rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(q.ctypes.data), q.shape[0], q.shape[1], k, 2.0,
+                             np.inf if md is None else md, 0.0, FP32_SCREEN | SCREEN_ONLY,
+                             ctypes.c_void_p(out_d.ctypes.data), ctypes.c_void_p(out_i.ctypes.data))
+    assert rc == 0, tree._lib.kdt_last_error().decode()
+    return out_d, out_i
+
+
+@pytest.mark.parametrize("case", CASES.values(), ids=CASES.keys())
+def test_screen_matches_oracle(case):
+    make_data, make_q, k, leafsize, md, (lo, hi) = case
+    data = np.ascontiguousarray(make_data())
+    q = np.ascontiguousarray(make_q())
+    od, oi = OracleTree(data, leafsize).query(q, k=k, max_distance=md)
+    tree = KDTree(data, leafsize=leafsize)
+    gd, gi = tree.query(q, k=k, max_distance=md, fp32_screen=True)
+    np.testing.assert_array_equal(gi, oi)
+    assert gd.tobytes() == od.tobytes()
+    # The plain FP64 path agrees too.
+    pd, pi = tree.query(q, k=k, max_distance=md)
+    np.testing.assert_array_equal(pi, oi)
+    assert pd.tobytes() == od.tobytes()
+    # Rows the matrix-core path answered by itself are exact as well.
+    sd, si = screen_only(tree, q, k, md)
+    done = si[:, 0] != -7
+    np.testing.assert_array_equal(si[done], oi[done])
+    assert sd[done].tobytes() == od[done].tobytes()
+    print(f"screen answered {done.mean():.3f} of the queries", flush=True)
+    assert lo <= done.mean() <= hi
+
+
+def test_screen_large_batch_device():
+    """2^14 queries on the device, many point slices per query tile."""
+    import torch
+
+    from kdtree import forest as F
+
+    data = f32_normal(21, 50_000, 64)
+    q = f32_normal(22, 1 << 14, 64)
+    od, oi = OracleTree(data, 32).query(q, k=10, threads=8)
+    tree = F.build(data, 32)
+    try:
+        qd = torch.from_numpy(q).cuda()
+        gd, gi = F.query_device(tree, qd, 10, flags=F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(gi.cpu().numpy(), oi)
+        assert gd.cpu().numpy().tobytes() == od.tobytes()
+    finally:
+        tree.free()

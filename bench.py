@@ -45,8 +45,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4"],
-                    help="c2 (default, the metric's configuration); c3 precision mode; c4 multi-tree batch")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2 (default, the metric's configuration); c3 precision mode; c4 multi-tree batch; "
+                         "c5 serving")
+    ap.add_argument("--requests", type=int, default=240, help="c5: requests per serving run")
+    ap.add_argument("--plain-seconds", type=float, default=40.0, help="c5: time budget of the plain sample")
     return ap.parse_args()
 
 
@@ -69,6 +72,8 @@ def main():
         sys.exit(2)
     if args.workload == "c3":
         return run_c3(args)
+    if args.workload == "c5":
+        return run_c5(args)
     if args.workload == "c4":
         return run_c4(args)
     import torch
@@ -299,6 +304,140 @@ def run_c4(args):
     }
     for tree, _, _ in items:
         tree.free()
+    print(json.dumps(out), flush=True)
+
+
+def run_c5(args):
+    """Serving: 32 resident trees (n log-uniform in [1e5, 8e6], d alternating
+    16 / 64, float32 standard normal), Poisson requests on uniform trees with
+    batch sizes log-uniform in [1, 65536], k = 10, 200 GB HBM admission budget.
+
+    plain   each request alone, one after another, through kdt_query (host
+            buffers in and out, the FP64 traversal kernel): what a caller of
+            the reference API gets from the plain GPU kernel;
+    served  the continuous-batching loop (kdtree.serve.Server): requests are
+            admitted while others run, coalesced into multi-tree launches on
+            several streams, FP32 screens (the matrix-core screen for d = 64,
+            and for d = 16 when it measures faster) with FP64 recheck.
+    Both give identical results; a sample is checked against the oracle and
+    against the plain path.  Latency = arrival to host-visible result."""
+    import torch
+
+    from kdtree import KDTree  # noqa: F401  (loads the library with torch's runtime first)
+    from kdtree import forest as F
+    from kdtree.serve import Request, Server, latency_stats, poisson_requests
+    from kdtree.workloads import C5, c5_queries, c5_specs
+    from oracle import OracleTree
+
+    lib = load_library()
+    specs = c5_specs()
+    dims = [d for _, d in specs]
+    k = C5["k"]
+    smallest = sorted(range(len(specs)), key=lambda t: specs[t][0])[:3]
+    trees, host_data = [], {}
+    t0 = time.perf_counter()
+    for t, (n, d) in enumerate(specs):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(C5["seed"] * 1000 + t)
+        x = torch.randn((n, d), generator=g, device="cuda", dtype=torch.float32).double()
+        trees.append(F.build(x, 32))
+        if t in smallest:
+            host_data[t] = x.cpu().numpy()
+        del x
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+
+    # d = 16: matrix-core screen or traversal, whichever is faster on a sample.
+    t16 = [t for t in range(len(specs)) if dims[t] == 16]
+    t16 = sorted(t16, key=lambda t: specs[t][0])[len(t16) // 2]
+    qs = torch.from_numpy(c5_queries(10**6, 4096, 16)).cuda()
+    tune = {}
+    for name, fl in (("traversal_fp32_screen", F.KDT_QUERY_FP32_SCREEN),
+                     ("matrix_screen", F.KDT_QUERY_FP32_SCREEN | 0x8)):
+        F.query_device(trees[t16], qs, k, flags=fl | F.KDT_QUERY_ASSUME_FINITE)
+        tune[name] = _time_steps(lambda: F.query_device(trees[t16], qs, k, flags=fl | F.KDT_QUERY_ASSUME_FINITE),
+                                 3, 1) * 1e3
+    serve_flags = F.KDT_QUERY_FP32_SCREEN | (0x8 if tune["matrix_screen"] < tune["traversal_fp32_screen"] else 0)
+
+    base = poisson_requests(args.requests, rate=1.0, tree_dims=dims, k=k, max_batch=C5["max_batch"],
+                            seed=C5["seed"], make_queries=c5_queries)
+
+    def fresh(rate, subset=None):
+        src = base if subset is None else subset
+        return [Request(r.rid, r.arrival / rate, r.tree, r.queries, r.k) for r in src]
+
+    # Plain: one request at a time, in arrival order, within the time budget;
+    # each request cut to at most 2048 queries (the served comparison on the
+    # same sample runs the same cut requests).
+    plain, plain_t, plain_q = [], 0.0, 0
+    for r0 in base:
+        r = Request(r0.rid, r0.arrival, r0.tree, np.ascontiguousarray(r0.queries[:2048]), r0.k)
+        out_d = np.empty((r.queries.shape[0], k))
+        out_i = np.empty((r.queries.shape[0], k), np.int64)
+        s0 = time.perf_counter()
+        rc = lib.kdt_query(trees[r.tree].handle, ctypes.c_void_p(r.queries.ctypes.data), r.queries.shape[0],
+                           r.queries.shape[1], k, 2.0, float("inf"), 0.0, 0, ctypes.c_void_p(out_d.ctypes.data),
+                           ctypes.c_void_p(out_i.ctypes.data))
+        dt = time.perf_counter() - s0
+        assert rc == 0, lib.kdt_last_error().decode()
+        plain.append((r, dt, out_d, out_i))
+        plain_t += dt
+        plain_q += r.queries.shape[0]
+        if plain_t > args.plain_seconds and len(plain) >= 4:
+            break
+    plain_lat = np.array([dt for _, dt, _, _ in plain]) * 1e3
+    sample = [r for r, _, _, _ in plain]
+
+    def serve(reqs):
+        server = Server(trees, n_streams=4, hbm_budget_bytes=C5["hbm_budget_bytes"],
+                        max_queries_per_launch=1 << 18, flags=serve_flags)
+        server.run(reqs, time_limit=600)
+        st = latency_stats(reqs)
+        st.update(launches=server.launches, max_inflight=server.max_inflight)
+        return st
+
+    # Served on the plain sample (same work), then the full stream at saturation
+    # (every request already waiting) and at fractions of that capacity.
+    same = fresh(1e12, sample)
+    served_sample = serve(same)
+    exact_vs_plain = all(np.array_equal(s.out_i, pi) and s.out_d.tobytes() == pd.tobytes()
+                         for s, (_, _, pd, pi) in zip(same, plain))
+    sat = serve(fresh(1e12))
+    cap_rps = sat["requests"] / sat["span_s"]
+    loads = {}
+    for frac in (0.5, 0.8):
+        loads[f"{frac:.1f}x_capacity"] = serve(fresh(cap_rps * frac))
+
+    parity = True
+    checked = 0
+    full = fresh(1e12)
+    serve(full)
+    for r in full:
+        if r.tree in host_data and checked < 4:
+            m = min(16, r.queries.shape[0])
+            od, oi = OracleTree(host_data[r.tree], 32).query(r.queries[:m], k=k, threads=8)
+            parity &= bool(np.array_equal(oi, r.out_i[:m]) and od.tobytes() == r.out_d[:m].tobytes())
+            checked += 1
+    out = {
+        "metric": "k-NN queries/s, served (continuous batching + FP32/matrix-core screen) vs plain GPU kernel, "
+                  "exact indices",
+        "value": sat["queries_per_s"], "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32 screen + f64 recheck", "data": "synthetic (torch.randn float32, seed 5000+t; SplitMix64 "
+                                                     "Box-Muller queries)",
+        "config": {"workload": "serve-32-trees-n1e5..8e6-d{16,64}-k10-batch1..65536-budget200GB",
+                   "n_points": int(sum(n for n, _ in specs)), "requests": args.requests},
+        "plain": {"queries_per_s": plain_q / plain_t, "requests": len(plain), "queries": plain_q,
+                  "note": "first requests in arrival order, each cut to <= 2048 queries",
+                  "p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99))},
+        "served_same_requests": served_sample,
+        "served_vs_plain_same_requests": served_sample["queries_per_s"] / (plain_q / plain_t),
+        "served_saturation": sat, "served_at_load": loads,
+        "d16_choice_ms_per_4096_queries": tune, "d16_matrix_screen": bool(serve_flags & 0x8),
+        "exact_vs_plain_bitwise": exact_vs_plain, "parity_vs_oracle_sample": parity, "oracle_checked_requests": checked,
+        "build_s": build_s,
+    }
+    for tr in trees:
+        tr.free()
     print(json.dumps(out), flush=True)
 
 

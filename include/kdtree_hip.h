@@ -63,6 +63,10 @@ enum kdt_error {
                                         more queries still take stream-ordered scratch
                                         (hipMallocAsync) for the leaf-order sort. */
 
+#define KDT_QUERY_MATRIX_SCREEN 0x8u /* with KDT_QUERY_FP32_SCREEN: the matrix-core
+                                        screen also for 16 <= ndim < 32, where the
+                                        traversal still prunes part of the tree (a
+                                        per-tree choice, made by timing both) */
 #define KDT_QUERY_SCREEN_ONLY 0x4u /* diagnostic, with KDT_QUERY_FP32_SCREEN on the
                                       matrix-core path: skip the traversal of the
                                       queries the certificate does not cover, whose

@@ -151,11 +151,13 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   int rc = query_params(p, max_distance, eps, base, msg);
   if (rc) return batch_fail(rc, msg);
   base.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
+  base.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
   const bool per_tree = (flags & KDT_BATCH_PER_TREE_LAUNCHES) != 0;
   const bool no_order = (flags & KDT_BATCH_NO_ORDER) != 0;
 
   // Validate; split items into width classes that run in shared launches and
-  // the rest (general p, other widths, k > 64), which run one launch each.
+  // the rest (general p, other widths, k > 64, the matrix-core screen), which
+  // run one launch sequence each.
   static const uint64_t widths[] = {1, 2, 3, 4, 8, 16};
   std::vector<uint64_t> cls_items[6];
   std::vector<uint64_t> single;
@@ -172,7 +174,10 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     int c = -1;
     for (int w = 0; w < 6; ++w)
       if (t.d == widths[w]) c = w;
-    if (per_tree || base.metric == M_LP || c < 0 || it.k > 64) single.push_back(i);
+    QueryParams probe = base;
+    probe.k = it.k;
+    probe.nq = it.n_queries;
+    if (per_tree || base.metric == M_LP || c < 0 || it.k > 64 || screen_applies(t, probe)) single.push_back(i);
     else cls_items[c].push_back(i);
   }
 

@@ -9,9 +9,9 @@
 //   idx    u32[n_points]               original row of tree position
 //   rows   f64[n_points][ndim]         rows in tree order (row-major), the
 //                                      reference's Tree::data
-//   rows32 f32[n_pad][stride]          32 <= ndim <= 64 only: rows in tree order
-//                                      rounded to f32, zero-padded to stride 32
-//                                      or 64 and to n_pad = 32 * ceil(n / 32)
+//   rows32 f32[n_pad][stride]          16 <= ndim <= 64 only: rows in tree order
+//                                      rounded to f32, zero-padded to stride 16,
+//                                      32 or 64 and to n_pad = 32 * ceil(n / 32)
 //                                      rows, for the matrix-core screen
 //   pnorm  f32[n_pad]                  |row32|^2 (+inf on padding rows)
 #pragma once
@@ -64,7 +64,8 @@ struct QueryParams {
   int metric = M_L2;
   double p = 2.0, cutoff = 0.0, eps_factor = 1.0;
   bool fp32_screen = false;
-  bool screen_only = false;  // diagnostic: no traversal after the matrix-core screen
+  bool screen_only = false;    // diagnostic: no traversal after the matrix-core screen
+  bool matrix_screen = false;  // matrix-core screen also for 16 <= ndim < 32
   bool allow_order = true;  // large batches may run in leaf order
 };
 
@@ -82,7 +83,7 @@ void set_last_error(std::string msg);
 int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
                  int64_t* d_idx, hipStream_t s, std::string& msg);
 
-// Matrix-core screen for 32 <= ndim <= 64 (kdt_screen.hip).
+// Matrix-core screen for 16 <= ndim <= 64 (kdt_screen.hip).
 struct KArgs;
 bool screen_applies(const Tree& t, const QueryParams& qp);
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg);

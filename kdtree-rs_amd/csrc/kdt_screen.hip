@@ -1,6 +1,6 @@
-// kdt_screen.hip -- exact k-NN for wide rows (32 <= ndim <= 64, p = 2,
-// eps = 0) by an MFMA distance screen, an FP64 recheck and a visit
-// certificate.
+// kdt_screen.hip -- exact k-NN for wide rows (32 <= ndim <= 64, and
+// 16 <= ndim < 32 on request; p = 2, eps = 0) by an MFMA distance screen, an
+// FP64 recheck and a visit certificate.
 //
 // Replaces, for these widths, the leaf scans of the reference's descent
 // (Streamed plan, src/kernel.rs:296-303) without changing any result.  At
@@ -367,8 +367,8 @@ int dev_alloc(T** p, size_t count, hipStream_t s) {
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.metric == M_L2 && qp.eps_factor == 1.0 && qp.k <= (uint64_t)KP && t.rows32 &&
-         qp.nq < (1ull << 31) &&
-         t.d >= 32 && t.d <= 64 && std::getenv("KDT_NO_MFMA_SCREEN") == nullptr;
+         qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64 &&
+         std::getenv("KDT_NO_MFMA_SCREEN") == nullptr;
 }
 
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg) {
@@ -417,8 +417,9 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   sa.n_tiles = n_tiles;
   sa.slices = (uint32_t)slices;
   const dim3 grid((uint32_t)qtiles, (uint32_t)slices);
-  const bool narrow = t.d <= 32;
-  if (narrow) k_screen<16, 1><<<grid, 64, 0, s>>>(sa);
+  const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;
+  if (ks == 8) k_screen<8, 1><<<grid, 64, 0, s>>>(sa);
+  else if (ks == 16) k_screen<16, 1><<<grid, 64, 0, s>>>(sa);
   else k_screen<32, 1><<<grid, 64, 0, s>>>(sa);
 
   TauArgs ta{};
@@ -436,7 +437,8 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   ta.max_abs = t.max_abs;
   k_screen_tau<<<(uint32_t)nq, 64, 0, s>>>(ta);
 
-  if (narrow) k_screen<16, 2><<<grid, 64, 0, s>>>(sa);
+  if (ks == 8) k_screen<8, 2><<<grid, 64, 0, s>>>(sa);
+  else if (ks == 16) k_screen<16, 2><<<grid, 64, 0, s>>>(sa);
   else k_screen<32, 2><<<grid, 64, 0, s>>>(sa);
 
   CheckArgs ca{};

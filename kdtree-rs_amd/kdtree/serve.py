@@ -71,8 +71,10 @@ class Server:
         self.launches = 0
 
     def _bytes(self, r: Request) -> int:
+        """Device bytes a request holds while in flight: queries, outputs and
+        the screen's scratch (per-slice k lists, candidate lists)."""
         m, d = r.queries.shape
-        return m * d * 8 + m * r.k * 16
+        return m * d * 8 + m * r.k * 16 + m * (256 * r.k + 320)
 
     def _dispatch(self, reqs, stream) -> _Launch:
         torch = self.torch

@@ -104,3 +104,23 @@ def c4_tree_inputs(t: int, n: int, d: int, q: int):
     data = uniform_f32(10_000 + t, 0, n * d).reshape(n, d)
     queries = uniform_f32(20_000 + t, 0, q * d).reshape(q, d)
     return data, queries
+
+
+# Serving: 32 resident trees, n log-uniform in [1e5, 8e6], d alternating
+# 16 / 64, float32 standard normal points (promoted to float64); requests
+# arrive as a Poisson stream, each on a uniform tree with a batch size
+# log-uniform in [1, 65536] and k = 10.
+C5 = dict(n_trees=32, seed=5, dims=(16, 64), k=10, max_batch=65536, hbm_budget_bytes=200e9)
+
+
+def c5_specs(n_trees: int = C5["n_trees"], scale: float = 1.0):
+    """(n, d) per tree; `scale` < 1 shrinks n for tests."""
+    u = uniform(C5["seed"], 0, n_trees)
+    return [(max(2, int(round(10 ** (5 + np.log10(80.0) * u[t]) * scale))), C5["dims"][t % 2])
+            for t in range(n_trees)]
+
+
+def c5_queries(request: int, m: int, d: int) -> np.ndarray:
+    """Standard normal float32 query rows of one request, promoted to float64."""
+    u = uniform(1_000_000 + request, 0, 2 * m * d)
+    return normal_from(u)[: m * d].astype(np.float32).astype(np.float64).reshape(m, d)

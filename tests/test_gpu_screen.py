@@ -21,6 +21,7 @@ pytestmark = pytest.mark.gpu
 
 FP32_SCREEN = 0x1
 SCREEN_ONLY = 0x4
+MATRIX_SCREEN = 0x8
 
 
 def f32_normal(seed: int, n: int, d: int) -> np.ndarray:
@@ -55,14 +56,18 @@ CASES = {
 }
 
 
-def screen_only(tree: KDTree, q: np.ndarray, k: int, md):
+def query_flags(tree: KDTree, q: np.ndarray, k: int, md, flags: int):
     out_d = np.full((q.shape[0], k), np.nan)
     out_i = np.full((q.shape[0], k), -7, dtype=np.int64)
     rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(q.ctypes.data), q.shape[0], q.shape[1], k, 2.0,
-                             np.inf if md is None else md, 0.0, FP32_SCREEN | SCREEN_ONLY,
+                             np.inf if md is None else md, 0.0, flags,
                              ctypes.c_void_p(out_d.ctypes.data), ctypes.c_void_p(out_i.ctypes.data))
     assert rc == 0, tree._lib.kdt_last_error().decode()
     return out_d, out_i
+
+
+def screen_only(tree: KDTree, q: np.ndarray, k: int, md, extra: int = 0):
+    return query_flags(tree, q, k, md, FP32_SCREEN | SCREEN_ONLY | extra)
 
 
 @pytest.mark.parametrize("case", CASES.values(), ids=CASES.keys())
@@ -86,6 +91,36 @@ def test_screen_matches_oracle(case):
     assert sd[done].tobytes() == od[done].tobytes()
     print(f"screen answered {done.mean():.3f} of the queries", flush=True)
     assert lo <= done.mean() <= hi
+
+
+NARROW = {
+    "normal-f32-20000x16-k10": (lambda: f32_normal(23, 20_000, 16), lambda: f32_normal(24, 700, 16), 10, 32),
+    "uniform-9000x16-k16-ls8": (lambda: np.random.default_rng(25).uniform(size=(9000, 16)),
+                                lambda: np.random.default_rng(26).uniform(size=(300, 16)), 16, 8),
+    "normal-4000x24-k3": (lambda: np.random.default_rng(27).normal(size=(4000, 24)),
+                          lambda: np.random.default_rng(28).normal(size=(129, 24)), 3, 32),
+}
+
+
+@pytest.mark.parametrize("case", NARROW.values(), ids=NARROW.keys())
+def test_matrix_screen_narrow_rows(case):
+    """16 <= ndim < 32 takes the matrix-core screen only when asked."""
+    make_data, make_q, k, leafsize = case
+    data = np.ascontiguousarray(make_data())
+    q = np.ascontiguousarray(make_q())
+    od, oi = OracleTree(data, leafsize).query(q, k=k)
+    tree = KDTree(data, leafsize=leafsize)
+    gd, gi = query_flags(tree, q, k, None, FP32_SCREEN | MATRIX_SCREEN)
+    np.testing.assert_array_equal(gi, oi)
+    assert gd.tobytes() == od.tobytes()
+    sd, si = screen_only(tree, q, k, None, MATRIX_SCREEN)
+    done = si[:, 0] != -7
+    np.testing.assert_array_equal(si[done], oi[done])
+    assert sd[done].tobytes() == od[done].tobytes()
+    assert done.mean() >= 0.95
+    # Without the request the traversal answers (nothing left unwritten).
+    _, ti = screen_only(tree, q, k, None)
+    np.testing.assert_array_equal(ti, oi)
 
 
 def test_screen_large_batch_device():

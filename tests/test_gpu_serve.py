@@ -1,5 +1,5 @@
 """The serving loop: requests that overlap on the device get exactly the result
-they get alone, and the oracle's."""
+they get alone (through the plain FP64 kernel), and the oracle's."""
 
 from __future__ import annotations
 
@@ -27,7 +27,9 @@ def test_served_results_equal_alone_and_oracle():
         return np.random.default_rng(1000 + i).standard_normal((m, d)).astype(np.float32).astype(np.float64)
 
     reqs = poisson_requests(40, rate=2000.0, tree_dims=dims, k=10, max_batch=400, seed=3, make_queries=make_q)
-    server = Server(trees, n_streams=3, max_queries_per_launch=600)
+    # Served with the FP32 screens (matrix-core screen for every tree here);
+    # compared below with the plain FP64 traversal of each request alone.
+    server = Server(trees, n_streams=3, max_queries_per_launch=600, flags=F.KDT_QUERY_FP32_SCREEN | 0x8)
     server.run(reqs, time_limit=120)
     stats = latency_stats(reqs)
     print(stats, "max_inflight", server.max_inflight, "launches", server.launches, flush=True)

@@ -45,9 +45,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "screen", "copy"],
                     help="c2 (default, the metric's configuration); c3 precision mode; c4 multi-tree batch; "
-                         "c5 serving")
+                         "c5 serving; screen: the matrix-core screen alone on one 64-d tree; copy: a 1 GiB "
+                         "device copy that calibrates the FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--requests", type=int, default=240, help="c5: requests per serving run")
     ap.add_argument("--plain-seconds", type=float, default=40.0, help="c5: time budget of the plain sample")
     return ap.parse_args()
@@ -74,6 +75,10 @@ def main():
         return run_c3(args)
     if args.workload == "c5":
         return run_c5(args)
+    if args.workload == "screen":
+        return run_screen(args)
+    if args.workload == "copy":
+        return run_copy(args)
     if args.workload == "c4":
         return run_c4(args)
     import torch
@@ -305,6 +310,83 @@ def run_c4(args):
     for tree, _, _ in items:
         tree.free()
     print(json.dumps(out), flush=True)
+
+
+def run_copy(args):
+    """3 copies of 1 GiB (device to device): 1 GiB read + 1 GiB written per
+    dispatch, the known traffic that scales the PMC byte counters."""
+    import torch
+
+    a = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_dispatch": 1 << 30, "bytes_written_per_dispatch": 1 << 30,
+                      "dispatches": 3}), flush=True)
+
+
+def run_screen(args):
+    """The matrix-core screen on its own: n = 2^20 float32 standard normal
+    rows of d = 64 (as float64), 65536 queries, k = 10.  Roofline: the FP32
+    matrix-core peak, with 2 d FLOPs per (query, point) pair as the
+    algorithmic work of one contraction (the screen makes two passes)."""
+    import torch
+
+    from kdtree import forest as F
+    from oracle import OracleTree
+
+    n, d, nq, k = 1 << 20, 64, 1 << 16, 10
+    g = torch.Generator(device="cuda")
+    g.manual_seed(64)
+    data = torch.randn((n, d), generator=g, device="cuda", dtype=torch.float32).double()
+    q = torch.randn((nq, d), generator=g, device="cuda", dtype=torch.float32).double()
+    tree = F.build(data, 32)
+    flags = F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_ASSUME_FINITE
+    od, oi = F.query_device(tree, q, k, flags=flags)
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    for _ in range(args.warmup):
+        F.query_device(tree, q, k, od, oi, flags=flags)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        starts[i].record(stream)
+        F.query_device(tree, q, k, od, oi, flags=flags)
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = np.array([s_.elapsed_time(e_) for s_, e_ in zip(starts, ends)])
+    flop = 2.0 * d * n * nq
+    achieved = flop / (ms.mean() / 1e3) / 1e12
+    m = 64
+    otree = OracleTree(data.cpu().numpy(), 32)
+    pd, pi = otree.query(q[:m].cpu().numpy(), k=k, threads=8)
+    parity = bool(np.array_equal(pi, oi[:m].cpu().numpy()) and pd.tobytes() == od[:m].cpu().numpy().tobytes())
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        qs = q[:256].cpu().numpy()
+        t1 = time.perf_counter()
+        otree.query(qs, k=k, threads=threads)
+        cpu = {"value": qs.shape[0] / (time.perf_counter() - t1), "unit": "queries/s", "cores": threads,
+               "kind": "port", "sample": f"256 of the {nq} queries, {threads} threads (oracle)"}
+    tree.free()
+    print(json.dumps({
+        "metric": "k-NN queries/s (exact indices), matrix-core screen, 64-d",
+        "value": nq * args.steps / wall, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "dtype": "f32 (matrix cores) + f64 recheck", "data": "synthetic (torch.randn float32, seed 64)",
+        "config": {"workload": "knn-normal-n1048576-d64-q65536-k10-leafsize32-p2"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": achieved / 157.3, "traffic": None,
+                     "what": "2 d FLOPs per (query, point) pair over the whole screened query call "
+                             "(two screen passes, candidate recheck, certificate)"},
+        "parity_sample_bit_exact": parity, "cpu_baseline": cpu,
+    }), flush=True)
 
 
 def run_c5(args):

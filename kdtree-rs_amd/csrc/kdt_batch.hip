@@ -63,10 +63,10 @@ __global__ void k_zero(unsigned long long* p, uint32_t n) {
 // order (or of the global numbering when `order` is null).  Groups take
 // queries from `counter` and run them through the phased descent; a group
 // switches to the query's item (tree and outputs) with each new query.
-template <int DT, int G, int M, bool SCREEN>
-__global__ __launch_bounds__(256) void knn_batch_kernel(const BItem* __restrict__ items, uint32_t n_items,
-                                                        const uint32_t* __restrict__ order, uint64_t first,
-                                                        uint64_t count, unsigned long long* __restrict__ counter) {
+template <int DT, int G, int M, bool SCREEN, int STK>
+__global__ __launch_bounds__(256, reg_min_waves<DT>()) void knn_batch_kernel(
+    const BItem* __restrict__ items, uint32_t n_items, const uint32_t* __restrict__ order, uint64_t first,
+    uint64_t count, unsigned long long* __restrict__ counter) {
   constexpr int GPB = 256 / G;
   __shared__ double s_sa[GPB][STK];
   __shared__ double s_sb[GPB][STK];
@@ -92,19 +92,18 @@ __global__ __launch_bounds__(256) void knn_batch_kernel(const BItem* __restrict_
   knn_ww<DT, G, M, SCREEN>(cur, best, s_ax[grp], s_q[grp], s_sa[grp], s_sb[grp], s_sn[grp], s_sm[grp], next);
 }
 
-template <int DT, int M>
-void launch_class(int G, bool screen, const BItem* items, uint32_t n, const uint32_t* order, uint64_t first,
-                  uint64_t count, unsigned long long* counter, hipStream_t s) {
-  const uint32_t blocks = 256u * 8u;  // resident grid: ~8 blocks per CU
+template <int DT, int M, int STK>
+void launch_class_stk(int G, bool screen, uint32_t blocks, const BItem* items, uint32_t n, const uint32_t* order,
+                      uint64_t first, uint64_t count, unsigned long long* counter, hipStream_t s) {
 #define KDT_CLASS(GG)                                                                                  \
   if (G == GG) {                                                                                       \
     if constexpr (M == M_L2) {                                                                         \
       if (screen) {                                                                                    \
-        knn_batch_kernel<DT, GG, M, true><<<blocks, 256, 0, s>>>(items, n, order, first, count, counter); \
+        knn_batch_kernel<DT, GG, M, true, STK><<<blocks, 256, 0, s>>>(items, n, order, first, count, counter); \
         return;                                                                                        \
       }                                                                                                \
     }                                                                                                  \
-    knn_batch_kernel<DT, GG, M, false><<<blocks, 256, 0, s>>>(items, n, order, first, count, counter); \
+    knn_batch_kernel<DT, GG, M, false, STK><<<blocks, 256, 0, s>>>(items, n, order, first, count, counter); \
     return;                                                                                            \
   }
   KDT_CLASS(8)
@@ -114,16 +113,24 @@ void launch_class(int G, bool screen, const BItem* items, uint32_t n, const uint
 #undef KDT_CLASS
 }
 
+template <int DT, int M>
+void launch_class(int G, bool screen, const BItem* items, uint32_t n, const uint32_t* order, uint64_t first,
+                  uint64_t count, unsigned long long* counter, uint32_t stk, hipStream_t s) {
+  const uint32_t blocks = 256u * 8u;  // resident grid: ~8 blocks per CU
+  if (stk <= 32) launch_class_stk<DT, M, 32>(G, screen, blocks, items, n, order, first, count, counter, s);
+  else launch_class_stk<DT, M, STK_MAX>(G, screen, blocks, items, n, order, first, count, counter, s);
+}
+
 template <int M>
 void launch_class_width(uint64_t d, int G, bool screen, const BItem* items, uint32_t n, const uint32_t* order,
-                        uint64_t first, uint64_t count, unsigned long long* counter, hipStream_t s) {
+                        uint64_t first, uint64_t count, unsigned long long* counter, uint32_t stk, hipStream_t s) {
   switch (d) {
-    case 1: launch_class<1, M>(G, screen, items, n, order, first, count, counter, s); break;
-    case 2: launch_class<2, M>(G, screen, items, n, order, first, count, counter, s); break;
-    case 3: launch_class<3, M>(G, screen, items, n, order, first, count, counter, s); break;
-    case 4: launch_class<4, M>(G, screen, items, n, order, first, count, counter, s); break;
-    case 8: launch_class<8, M>(G, screen, items, n, order, first, count, counter, s); break;
-    case 16: launch_class<16, M>(G, screen, items, n, order, first, count, counter, s); break;
+    case 1: launch_class<1, M>(G, screen, items, n, order, first, count, counter, stk, s); break;
+    case 2: launch_class<2, M>(G, screen, items, n, order, first, count, counter, stk, s); break;
+    case 3: launch_class<3, M>(G, screen, items, n, order, first, count, counter, stk, s); break;
+    case 4: launch_class<4, M>(G, screen, items, n, order, first, count, counter, stk, s); break;
+    case 8: launch_class<8, M>(G, screen, items, n, order, first, count, counter, stk, s); break;
+    case 16: launch_class<16, M>(G, screen, items, n, order, first, count, counter, stk, s); break;
   }
 }
 
@@ -195,6 +202,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   // contiguous range of the batch's query numbers (and of the sorted order).
   std::vector<BItem> h;
   uint64_t first[6] = {0}, count[6] = {0}, kmax[6] = {0};
+  uint32_t stk[6] = {2, 2, 2, 2, 2, 2};
   uint64_t total = 0;
   for (int c = 0; c < 6; ++c) {
     first[c] = total;
@@ -208,6 +216,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
       b.q_begin = total;
       total += it.n_queries;
       kmax[c] = std::max<uint64_t>(kmax[c], it.k);
+      stk[c] = std::max<uint32_t>(stk[c], b.a.stk);
       h.push_back(b);
     }
     count[c] = total - first[c];
@@ -258,13 +267,13 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
       const int G = kmax[c] <= 16 ? 16 : kmax[c] <= 32 ? 32 : 64;  // G = 8 measured no faster
       switch (base.metric) {
         case M_L1:
-          launch_class_width<M_L1>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, s);
+          launch_class_width<M_L1>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
           break;
         case M_L2:
-          launch_class_width<M_L2>(widths[c], G, screen, d_items, n, order, first[c], count[c], counters + c, s);
+          launch_class_width<M_L2>(widths[c], G, screen, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
           break;
         default:
-          launch_class_width<M_LINF>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, s);
+          launch_class_width<M_LINF>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
           break;
       }
     }

@@ -122,6 +122,7 @@ int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double*
   a.screen_gam = (1.0 + D * u / (1.0 - D * u)) * (1.0 + 1e-14);
   a.screen_shrink = 1.0 + 2.0 * (D + 3.0) * std::ldexp(1.0, -53);
   a.max_abs = t.max_abs;
+  a.stk = std::max<uint32_t>(2, 2 * t.depth);
   return 0;
 }
 

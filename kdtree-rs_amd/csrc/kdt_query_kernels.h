@@ -47,10 +47,24 @@
 
 namespace kdt {
 
-// Stack entries per group.  The build checks every node (validate_nodes:
-// preorder child ids, split dims, leaf ranges) and that the depth is at most
-// 32, so the traversal reads nodes unchecked and never overflows the stack.
-constexpr int STK = 64;
+// Stack entries per group: every inner node on the current path holds at
+// most one deferred entry and one RESTORE, so 2 x depth entries suffice; the
+// build checks every node (k_validate_nodes: preorder child ids, split dims,
+// leaf ranges) and that the depth is at most 32, so the traversal reads nodes
+// unchecked and never overflows its stack.  The register-list kernels take
+// 32 entries for trees of depth <= 16 (less LDS, more resident waves) and
+// STK_MAX otherwise; the runtime-width kernel keeps STK_MAX.
+constexpr int STK_MAX = 64;
+
+// Waves per SIMD the register-list kernels are compiled for.  The descent is
+// bound by the latency of its dependent node loads more than by issue, so
+// occupancy pays: rows of 1..4 coordinates run 8 waves (a few registers of
+// the query start-up spill), wider rows 6 (they spill in their hot loops at 8).
+template <int DT>
+constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : 6; }
+// Hits in one leaf chunk from which a group inserts them with one bitonic
+// merge instead of one by one (the merge costs about ten single inserts).
+constexpr uint32_t BULK_MIN = 10;
 constexpr uint32_t KIND_FAR = 0, KIND_BOXFAR = 1, KIND_BOXNEAR = 2, KIND_RESTORE = 3;
 
 struct KArgs {
@@ -63,6 +77,7 @@ struct KArgs {
   const double* __restrict__ q;
   const uint32_t* __restrict__ order;  // processing order of the queries (null: as given)
   const uint8_t* __restrict__ only;    // non-null: run only the queries qi with only[qi] != 0
+  uint32_t stk;                        // stack entries per group (2 x tree depth, at least 2)
   double* __restrict__ out_d;
   int64_t* __restrict__ out_i;
   uint64_t nq;
@@ -556,7 +571,7 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
         const uint32_t myidx = hit ? a.idx[start + o] : 0u;
         if constexpr (List::kBulk) {
           const uint32_t nv = (uint32_t)__popcll(m);
-          if (nv >= 4) {
+          if (nv >= BULK_MIN) {
             bound = best.insert_many(dist, myidx, hit && dist <= bound, nv, bound, lane, a);
             if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
             return;
@@ -582,36 +597,49 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
       }
       state = ST_POP;
     }
-    // 3. unwind to the next deferred visit
+    // 3. unwind to the next deferred visit.  RESTORE entries and far children
+    // whose plane bound already fails need no box distance: each group
+    // consumes those in its own inner loop, so one wave iteration costs at
+    // most one box gate per group.  Order and bound reads are unchanged.
     for (;;) {
+      bool test = false;
+      uint32_t en = 0, kind = 0, edim = 0;
+      double ea = 0.0, eb = 0.0;
       if (alive && state == ST_POP) {
-        if (sp == 0) {
-          state = ST_DONE;
-        } else {
+        for (;;) {
+          if (sp == 0) {
+            state = ST_DONE;
+            break;
+          }
           --sp;
-          const uint32_t meta = s_sm[sp], en = s_sn[sp];
-          const double ea = s_sa[sp], eb = s_sb[sp];
-          const uint32_t kind = meta & 3u, edim = meta >> 2;
+          const uint32_t meta = s_sm[sp];
+          kind = meta & 3u;
+          edim = meta >> 2;
+          ea = s_sa[sp];
           if (kind == KIND_RESTORE) {
             axes[edim] = ea;
-          } else if (kind == KIND_BOXNEAR) {
-            if (box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound) {
-              node = en;
-              cell = ea;
-              state = ST_DESCEND;
-            }
-          } else {
-            bool ok = kind == KIND_BOXFAR || eb * eps <= bound;
-            if (ok) ok = box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound;
-            if (ok) {
-              s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
-              ++sp;
-              axes[edim] = ea;
-              node = en;
-              cell = eb;
-              state = ST_DESCEND;
-            }
+            continue;
           }
+          eb = s_sb[sp];
+          if (kind == KIND_FAR && !(eb * eps <= bound)) continue;
+          en = s_sn[sp];
+          test = true;
+          break;
+        }
+      }
+      if (test) {
+        if (box_dist_grp<M, DT, G>(q, s_q, a.boxes + (size_t)en * 2 * D, D, a.p, lane) * eps <= bound) {
+          if (kind == KIND_BOXNEAR) {
+            node = en;
+            cell = ea;
+          } else {  // enter_far, src/query.rs:329-341
+            s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
+            ++sp;
+            axes[edim] = ea;
+            node = en;
+            cell = eb;
+          }
+          state = ST_DESCEND;
         }
       }
       if (!__ballot(alive && state == ST_POP)) break;
@@ -649,8 +677,9 @@ __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s
   knn_ww<DT, G, M, SCREEN>(a, best, axes, s_q, s_sa, s_sb, s_sn, s_sm, next);
 }
 
-template <int DT, int G, int M, bool SCREEN>
-__global__ __launch_bounds__(256) void knn_reg_kernel(KArgs a) {
+// STK: stack entries per group, 32 for trees of depth <= 16, else 64.
+template <int DT, int G, int M, bool SCREEN, int STK>
+__global__ __launch_bounds__(256, reg_min_waves<DT>()) void knn_reg_kernel(KArgs a) {
   constexpr int GPB = 256 / G;
   constexpr int DL = DT > 0 ? DT : 1;
   __shared__ double s_sa[GPB][STK];
@@ -669,10 +698,10 @@ __global__ __launch_bounds__(256) void knn_reg_kernel(KArgs a) {
 template <int M>
 __global__ __launch_bounds__(64) void knn_lds_kernel(KArgs a) {
   extern __shared__ double dyn[];
-  __shared__ double s_sa[STK];
-  __shared__ double s_sb[STK];
-  __shared__ uint32_t s_sn[STK];
-  __shared__ uint32_t s_sm[STK];
+  __shared__ double s_sa[STK_MAX];
+  __shared__ double s_sb[STK_MAX];
+  __shared__ uint32_t s_sn[STK_MAX];
+  __shared__ uint32_t s_sm[STK_MAX];
   double* axes = dyn;
   double* l_d = dyn + a.D;
   uint32_t* l_i = reinterpret_cast<uint32_t*>(l_d + a.k);
@@ -694,7 +723,8 @@ inline void launch_reg(const KArgs& a, hipStream_t s) {
   if (const char* env = std::getenv("KDT_QUERY_BLOCKS")) cap = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
   uint64_t blocks = (a.nq + GPB - 1) / GPB;
   if (blocks > cap) blocks = cap;
-  knn_reg_kernel<DT, G, M, SCREEN><<<(uint32_t)blocks, 256, 0, s>>>(a);
+  if (a.stk <= 32) knn_reg_kernel<DT, G, M, SCREEN, 32><<<(uint32_t)blocks, 256, 0, s>>>(a);
+  else knn_reg_kernel<DT, G, M, SCREEN, STK_MAX><<<(uint32_t)blocks, 256, 0, s>>>(a);
 }
 
 // Widths with a compiled kernel: the reference's own monomorphised list

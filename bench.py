@@ -332,7 +332,7 @@ def run_screen(args):
     """The matrix-core screen on its own: n = 2^20 float32 standard normal
     rows of d = 64 (as float64), 65536 queries, k = 10.  Roofline: the FP32
     matrix-core peak, with 2 d FLOPs per (query, point) pair as the
-    algorithmic work of one contraction (the screen makes two passes)."""
+    algorithmic work of the contraction."""
     import torch
 
     from kdtree import forest as F
@@ -384,7 +384,7 @@ def run_screen(args):
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": 157.3, "unit": "TFLOP/s",
                      "frac": achieved / 157.3, "traffic": None,
                      "what": "2 d FLOPs per (query, point) pair over the whole screened query call "
-                             "(two screen passes, candidate recheck, certificate)"},
+                             "(matrix-core screen, candidate selection, FP64 recheck, certificate)"},
         "parity_sample_bit_exact": parity, "cpu_baseline": cpu,
     }), flush=True)
 

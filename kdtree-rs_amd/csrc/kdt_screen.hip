@@ -10,15 +10,18 @@
 // contraction, which is what the matrix cores do.
 //
 // Steps for a query matrix against one tree:
-//   1. screen pass 1 (k_screen<.., 1>): for a tile of 32 queries and a slice
-//      of the points, v_mfma_f32_32x32x2f32 forms q.p in FP32 and each lane
-//      keeps the k smallest  A = |q|^2 + |p|^2 - 2 q.p  of the points it sees;
-//   2. k_screen_tau: the k-th smallest A over all slices gives A_k, and from
-//      it a threshold tau_A such that every point whose FP64 distance could
-//      be among the k best has A <= tau_A (error bound below);
-//   3. screen pass 2 (k_screen<.., 2>): the same products again; points with
-//      A <= tau_A are appended to the query's candidate list;
-//   4. k_screen_recheck: FP64 distances of the candidates in the reference's
+//   1. k_screen: for a tile of 32 queries and a slice of the points,
+//      v_mfma_f32_32x32x2f32 forms q.p in FP32 and each lane keeps the KP = 16
+//      smallest  A = |q|^2 + |p|^2 - 2 q.p  of the points it sees, with the
+//      points' tree positions;
+//   2. k_screen_select: the k-th smallest A over all of a query's lists gives
+//      A_k, and from it a threshold tau_A such that every point whose FP64
+//      distance could be among the k best has A <= tau_A (error bound
+//      below).  A lane dropped only points with A >= its final 16th value, so
+//      when every full list's 16th value exceeds tau_A the list entries with
+//      A <= tau_A are all the candidates; otherwise (or beyond 64
+//      candidates) the query is flagged;
+//   3. k_screen_recheck: FP64 distances of the candidates in the reference's
 //      summation order (point_dist_d), lexicographic (distance, index) top-k,
 //      then the certificate: for each of the k points, every gate on its
 //      root-to-leaf path (plane bound and box bound of each far child, box
@@ -26,9 +29,8 @@
 //      is <= the k-th distance.  The reference's bound never drops below the
 //      k-th distance, so it opens every one of those gates: it visits every
 //      point of this top-k, and its answer (the top-k of what it visits) is
-//      this top-k.  Queries whose candidates overflow, or whose certificate
-//      fails, are flagged;
-//   5. the traversal kernels run the flagged queries (KArgs::only).
+//      this top-k.  Queries whose certificate fails are flagged;
+//   4. the traversal kernels run the flagged queries (KArgs::only).
 //
 // FP32 error bound (u = 2^-24, q~ and p~ the rows rounded to FP32, P the
 // largest |p~| of the tree, N = |q~| + P):
@@ -66,12 +68,10 @@ struct SArgs {
   const float* __restrict__ rows32;  // [n_pad][2 KS], tree order, zero-padded
   const float* __restrict__ pnorm;   // [n_pad] |p~|^2, +inf on padding rows
   const double* __restrict__ q;      // [nq][D]
-  float* __restrict__ part;          // pass 1: [nq][2 S][k]
-  const float* __restrict__ tau;     // pass 2: [nq]
-  uint32_t* __restrict__ cnt;        // pass 2: [nq]
-  uint32_t* __restrict__ cand;       // pass 2: [nq][CAND_CAP]
+  float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
+  uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
   uint64_t nq;
-  uint32_t D, k, n, tiles_per_slice, n_tiles, slices;
+  uint32_t D, n, tiles_per_slice, n_tiles, slices;
 };
 
 // One wave: 32 queries (B operand, columns) against 32-row tiles of points
@@ -79,7 +79,7 @@ struct SArgs {
 // [h KS, h KS + KS) with h = l >> 5, in both operands, so instruction s
 // contracts coordinates s and KS + s.  The accumulator holds D[i][j] for
 // point rows i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
-template <int KS, int PASS>
+template <int KS>
 __global__ __launch_bounds__(64) void k_screen(SArgs a) {
   const uint32_t lane = threadIdx.x;
   const uint32_t j = lane & 31, h = lane >> 5;
@@ -98,14 +98,14 @@ __global__ __launch_bounds__(64) void k_screen(SArgs a) {
   const uint32_t t0 = blockIdx.y * a.tiles_per_slice;
   const uint32_t t1 = min(a.n_tiles, t0 + a.tiles_per_slice);
 
-  float list[KP];  // PASS 1: KP - k sentinels (-inf) then the k smallest A, ascending
-  float thr = INFINITY;
-  if constexpr (PASS == 1) {
+  float la[KP];     // the KP smallest A seen by this lane, ascending
+  uint32_t li[KP];  // their tree positions
 #pragma unroll
-    for (int e = 0; e < KP; ++e) list[e] = (e < KP - (int)a.k) ? -INFINITY : INFINITY;
-  } else {
-    thr = qvalid ? a.tau[qrow] : -INFINITY;
+  for (int e = 0; e < KP; ++e) {
+    la[e] = INFINITY;
+    li[e] = 0xFFFFFFFFu;
   }
+  float thr = INFINITY;  // la[KP - 1]
 
   constexpr int NV = KS / 4;
   float4 cur[NV], nxt[NV];
@@ -133,20 +133,21 @@ __global__ __launch_bounds__(64) void k_screen(SArgs a) {
     for (int r = 0; r < 16; ++r) {
       const float4 pg = pn_cur[r / 4];
       const float pn = (r % 4 == 0) ? pg.x : (r % 4 == 1) ? pg.y : (r % 4 == 2) ? pg.z : pg.w;
-      const float A = fmaf(-2.0f, acc[r], qn + pn);
-      if constexpr (PASS == 1) {
-        if (A < thr) {  // insert, dropping the largest
-#pragma unroll
-          for (int e = KP - 1; e > 0; --e) list[e] = fminf(list[e], fmaxf(list[e - 1], A));
-          list[0] = fminf(list[0], A);
-          thr = list[KP - 1];
-        }
-      } else {
+      const float A = fmaf(-2.0f, acc[r], qn + pn);  // +inf on padding rows
+      if (A < thr) {  // insert (A, row), dropping the largest
         const uint32_t row = t * 32 + 8 * (r / 4) + 4 * h + (r % 4);
-        if (A <= thr && row < a.n) {
-          const uint32_t c = atomicAdd(&a.cnt[qrow], 1u);
-          if (c < CAND_CAP) a.cand[qrow * CAND_CAP + c] = row;
+#pragma unroll
+        for (int e = KP - 1; e > 0; --e) {
+          const bool up = A < la[e - 1];
+          const bool here = !up && A < la[e];
+          la[e] = up ? la[e - 1] : (here ? A : la[e]);
+          li[e] = up ? li[e - 1] : (here ? row : li[e]);
         }
+        if (A < la[0]) {
+          la[0] = A;
+          li[0] = row;
+        }
+        thr = la[KP - 1];
       }
     }
 #pragma unroll
@@ -154,37 +155,39 @@ __global__ __launch_bounds__(64) void k_screen(SArgs a) {
 #pragma unroll
     for (int g = 0; g < 4; ++g) pn_cur[g] = pn_nxt[g];
   }
-  if constexpr (PASS == 1) {
-    if (qvalid) {
-      float* out = a.part + (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * a.k;
+  if (qvalid) {
+    const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
 #pragma unroll
-      for (int e = 0; e < KP; ++e)
-        if (e >= KP - (int)a.k) out[e - (KP - (int)a.k)] = list[e];
+    for (int e = 0; e < KP; ++e) {
+      a.part_a[base + e] = la[e];
+      a.part_i[base + e] = li[e];
     }
   }
 }
 
-struct TauArgs {
-  const float* __restrict__ part;
+struct SelArgs {
+  const float* __restrict__ part_a;
+  const uint32_t* __restrict__ part_i;
   const double* __restrict__ q;
-  float* __restrict__ tau;
   uint32_t* __restrict__ cnt;
+  uint32_t* __restrict__ cand;
   uint8_t* __restrict__ flag;
   uint64_t nq;
   uint32_t D, k, lists;
   double pmax, cutoff, max_abs;
 };
 
-// One wave per query: k rounds of wave-min over the heads of the sorted
-// per-lane lists give A_k; then tau_A from the error bound.  Also clears the
-// query's candidate count and fallback flag.
-__global__ __launch_bounds__(64) void k_screen_tau(TauArgs a) {
+// One wave per query, lane l reading list l: k rounds of wave-min over the
+// list heads give A_k, then tau_A from the error bound, the completeness
+// check and the candidates (list entries with A <= tau_A, at most CAND_CAP).
+__global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
-  const float* lst = a.part + (qi * a.lists + lane) * a.k;
-  uint32_t head = 0;
   const bool mine = lane < a.lists;
-  float cur = mine ? lst[0] : INFINITY;
+  const float* la = a.part_a + (qi * a.lists + lane) * KP;
+  const uint32_t* li = a.part_i + (qi * a.lists + lane) * KP;
+  uint32_t head = 0;
+  float cur = mine ? la[0] : INFINITY;
   float ak = INFINITY;
   for (uint32_t r = 0; r < a.k; ++r) {
     float m = cur;
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(64) void k_screen_tau(TauArgs a) {
     const uint64_t hit = __ballot(mine && cur == m);
     if (hit && lane == (uint32_t)(__ffsll((unsigned long long)hit) - 1)) {
       ++head;
-      cur = head < a.k ? lst[head] : INFINITY;
+      cur = head < KP ? la[head] : INFINITY;
     }
   }
   double qn = 0.0, qmax = 0.0;
@@ -209,31 +212,54 @@ __global__ __launch_bounds__(64) void k_screen_tau(TauArgs a) {
     qn += __shfl_xor(qn, o);
     qmax = fmax(qmax, __shfl_xor(qmax, o));
   }
+  // tau_A (every lane computes the same value)
+  const double u = 0x1p-24;
+  const double D = (double)a.D;
+  const double m2 = (2.0 * D + 4.0) * u;
+  const double g = m2 / (1.0 - m2);
+  const double N = (sqrt(qn) * (1.0 + 0x1p-50) + a.pmax);
+  const double E2 = ((2.0 * g + 8.0 * u) * N * N + D * 0x1p-125) * (1.0 + 1e-12);
+  const double e1 = (u * (1.0 + 2.0 * u) * N + sqrt(D) * 0x1p-148) * (1.0 + 1e-12);
+  const double sh = 1.0 + 2.0 * (D + 3.0) * 0x1p-53;
+  double tau = a.cutoff;
+  if (ak < INFINITY) {
+    const double r = sqrt(fmax(0.0, (double)ak + E2)) + e1;
+    tau = fmin(tau, sh * r * r * (1.0 + 1e-12));
+  }
+  float ta = INFINITY;
+  if (tau < INFINITY) {
+    const double r = sqrt(sh * tau) + e1;
+    const double t = (r * r + E2) * (1.0 + 1e-12);
+    ta = (float)t;
+    if ((double)ta < t) ta = nextafterf(ta, INFINITY);
+  }
+  // Completeness: a full list whose last value is <= tau_A may have dropped
+  // a candidate.
+  uint32_t mcount = 0;
+  bool complete = true;
+  if (mine) {
+    const float last = la[KP - 1];
+    complete = last == INFINITY || last > ta;
+    // Unfilled slots (A = +inf, position 0xFFFFFFFF) end the sorted list.
+    while (mcount < KP && li[mcount] != 0xFFFFFFFFu && la[mcount] <= ta) ++mcount;
+  }
+  const bool ok = qmax <= 1e18 && a.max_abs <= 1e18 && !__ballot(!complete);
+  // Exclusive prefix sum of the per-list counts.
+  uint32_t incl = mcount;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  const bool good = ok && total <= CAND_CAP;
+  if (good) {
+    const uint32_t off = incl - mcount;
+    for (uint32_t e = 0; e < mcount; ++e) a.cand[qi * CAND_CAP + off + e] = li[e];
+  }
   if (lane == 0) {
-    const double u = 0x1p-24;
-    const double D = (double)a.D;
-    const double m2 = (2.0 * D + 4.0) * u;
-    const double g = m2 / (1.0 - m2);
-    const double N = (sqrt(qn) * (1.0 + 0x1p-50) + a.pmax);
-    const double E2 = ((2.0 * g + 8.0 * u) * N * N + D * 0x1p-125) * (1.0 + 1e-12);
-    const double e1 = (u * (1.0 + 2.0 * u) * N + sqrt(D) * 0x1p-148) * (1.0 + 1e-12);
-    const double sh = 1.0 + 2.0 * (D + 3.0) * 0x1p-53;
-    double tau = a.cutoff;
-    if (ak < INFINITY) {
-      const double r = sqrt(fmax(0.0, (double)ak + E2)) + e1;
-      tau = fmin(tau, sh * r * r * (1.0 + 1e-12));
-    }
-    float t32 = INFINITY;
-    if (tau < INFINITY) {
-      const double r = sqrt(sh * tau) + e1;
-      const double ta = (r * r + E2) * (1.0 + 1e-12);
-      t32 = (float)ta;
-      if ((double)t32 < ta) t32 = nextafterf(t32, INFINITY);
-    }
-    const bool ok = qmax <= 1e18 && a.max_abs <= 1e18;
-    a.tau[qi] = ok ? t32 : -INFINITY;
-    a.flag[qi] = ok ? 0 : 1;
-    a.cnt[qi] = 0;
+    a.cnt[qi] = good ? total : 0;
+    a.flag[qi] = good ? 0 : 1;
   }
 }
 
@@ -273,8 +299,16 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   const double* q = a.q + qi * D;
   double d = INFINITY;
   uint32_t id = 0xFFFFFFFFu, pos = 0;
+  bool bad_pos = false;
   if (lane < c) {
     pos = a.cand[qi * CAND_CAP + lane];
+    bad_pos = pos >= a.n;
+  }
+  if (__ballot(bad_pos)) {  // never for a list built by k_screen_select
+    if (lane == 0) a.flag[qi] = 1;
+    return;
+  }
+  if (lane < c) {
     const double v = point_dist_d<M_L2, 0>(q, a.rows + (size_t)pos * D, 1, D, true, 2.0);
     if (v <= a.cutoff) {
       d = v;
@@ -383,17 +417,17 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   const uint32_t per = (uint32_t)((n_tiles + slices - 1) / slices);
   slices = (n_tiles + per - 1) / per;
 
-  float *part = nullptr, *tau = nullptr;
-  uint32_t *cnt = nullptr, *cand = nullptr;
+  float* part_a = nullptr;
+  uint32_t *part_i = nullptr, *cnt = nullptr, *cand = nullptr;
   uint8_t* flag = nullptr;
   int bad = 0;
-  bad |= dev_alloc(&part, nq * 2 * slices * k, s);
-  bad |= dev_alloc(&tau, nq, s);
+  bad |= dev_alloc(&part_a, nq * 2 * slices * KP, s);
+  bad |= dev_alloc(&part_i, nq * 2 * slices * KP, s);
   bad |= dev_alloc(&cnt, nq, s);
   bad |= dev_alloc(&cand, nq * CAND_CAP, s);
   bad |= dev_alloc(&flag, nq, s);
   auto release = [&]() {
-    for (void* p : {(void*)part, (void*)tau, (void*)cnt, (void*)cand, (void*)flag})
+    for (void* p : {(void*)part_a, (void*)part_i, (void*)cnt, (void*)cand, (void*)flag})
       if (p) (void)hipFreeAsync(p, s);
   };
   if (bad) {
@@ -405,41 +439,35 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   sa.rows32 = t.rows32;
   sa.pnorm = t.pnorm;
   sa.q = trav.q;
-  sa.part = part;
-  sa.tau = tau;
-  sa.cnt = cnt;
-  sa.cand = cand;
+  sa.part_a = part_a;
+  sa.part_i = part_i;
   sa.nq = nq;
   sa.D = (uint32_t)t.d;
-  sa.k = k;
   sa.n = (uint32_t)t.n;
   sa.tiles_per_slice = per;
   sa.n_tiles = n_tiles;
   sa.slices = (uint32_t)slices;
   const dim3 grid((uint32_t)qtiles, (uint32_t)slices);
   const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;
-  if (ks == 8) k_screen<8, 1><<<grid, 64, 0, s>>>(sa);
-  else if (ks == 16) k_screen<16, 1><<<grid, 64, 0, s>>>(sa);
-  else k_screen<32, 1><<<grid, 64, 0, s>>>(sa);
+  if (ks == 8) k_screen<8><<<grid, 64, 0, s>>>(sa);
+  else if (ks == 16) k_screen<16><<<grid, 64, 0, s>>>(sa);
+  else k_screen<32><<<grid, 64, 0, s>>>(sa);
 
-  TauArgs ta{};
-  ta.part = part;
-  ta.q = trav.q;
-  ta.tau = tau;
-  ta.cnt = cnt;
-  ta.flag = flag;
-  ta.nq = nq;
-  ta.D = (uint32_t)t.d;
-  ta.k = k;
-  ta.lists = (uint32_t)(2 * slices);
-  ta.pmax = t.pmax;
-  ta.cutoff = qp.cutoff;
-  ta.max_abs = t.max_abs;
-  k_screen_tau<<<(uint32_t)nq, 64, 0, s>>>(ta);
-
-  if (ks == 8) k_screen<8, 2><<<grid, 64, 0, s>>>(sa);
-  else if (ks == 16) k_screen<16, 2><<<grid, 64, 0, s>>>(sa);
-  else k_screen<32, 2><<<grid, 64, 0, s>>>(sa);
+  SelArgs se{};
+  se.part_a = part_a;
+  se.part_i = part_i;
+  se.q = trav.q;
+  se.cnt = cnt;
+  se.cand = cand;
+  se.flag = flag;
+  se.nq = nq;
+  se.D = (uint32_t)t.d;
+  se.k = k;
+  se.lists = (uint32_t)(2 * slices);
+  se.pmax = t.pmax;
+  se.cutoff = qp.cutoff;
+  se.max_abs = t.max_abs;
+  k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(se);
 
   CheckArgs ca{};
   ca.q = trav.q;

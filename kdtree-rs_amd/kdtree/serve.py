@@ -9,7 +9,8 @@ time.  The dispatcher loop:
     pinned host memory) and frees their share of the HBM admission budget;
   * while a stream is free and the budget allows, takes pending requests in
     arrival order up to the per-launch query cap and dispatches them as ONE
-    multi-tree launch (kdt_query_batch_device) on that stream: queries go
+    multi-tree launch (kdt_query_batch_device) on that stream: the requests
+    on one tree (with one k) are concatenated into one item, queries go
     host->device, the launch runs, results go device->host, then an event.
 
 Requests that arrive while earlier launches run are admitted at the next
@@ -47,9 +48,7 @@ class Request:
 class _Launch:
     stream: object
     event: object
-    reqs: list
-    host_d: list
-    host_i: list
+    groups: list  # (requests, pinned host distances, pinned host indices), one per item
     dev: list = field(default_factory=list)
     nbytes: int = 0
 
@@ -78,32 +77,35 @@ class Server:
 
     def _dispatch(self, reqs, stream) -> _Launch:
         torch = self.torch
-        items = (F.BatchItem * len(reqs))()
-        dev, host_d, host_i = [], [], []
+        by_item: dict = {}
+        for r in reqs:  # requests on one tree with one k share an item
+            by_item.setdefault((r.tree, r.k), []).append(r)
+        items = (F.BatchItem * len(by_item))()
+        dev, groups = [], []
         with torch.cuda.stream(stream):
-            for j, r in enumerate(reqs):
-                m, d = r.queries.shape
-                qh = torch.from_numpy(r.queries).pin_memory()
+            for j, ((tree_id, k), rs) in enumerate(by_item.items()):
+                q = rs[0].queries if len(rs) == 1 else np.concatenate([r.queries for r in rs])
+                m = q.shape[0]
+                qh = torch.from_numpy(q).pin_memory()
                 qd = qh.to("cuda", non_blocking=True)
-                od = torch.empty((m, r.k), dtype=torch.float64, device="cuda")
-                oi = torch.empty((m, r.k), dtype=torch.int64, device="cuda")
-                t = self.trees[r.tree]
-                items[j] = F.BatchItem(t.handle.value, qd.data_ptr(), m, t.d, r.k, od.data_ptr(), oi.data_ptr())
+                od = torch.empty((m, k), dtype=torch.float64, device="cuda")
+                oi = torch.empty((m, k), dtype=torch.int64, device="cuda")
+                t = self.trees[tree_id]
+                items[j] = F.BatchItem(t.handle.value, qd.data_ptr(), m, t.d, k, od.data_ptr(), oi.data_ptr())
                 dev.append((qh, qd, od, oi))
-            rc = self.lib.kdt_query_batch_device(items, len(reqs), self.p, float("inf"), 0.0, self.flags,
+            rc = self.lib.kdt_query_batch_device(items, len(by_item), self.p, float("inf"), 0.0, self.flags,
                                                  ctypes.c_void_p(stream.cuda_stream))
             if rc:
                 raise RuntimeError(self.lib.kdt_last_error().decode())
-            for (_, _, od, oi) in dev:
+            for rs, (_, _, od, oi) in zip(by_item.values(), dev):
                 hd = torch.empty(od.shape, dtype=torch.float64, pin_memory=True)
                 hi = torch.empty(oi.shape, dtype=torch.int64, pin_memory=True)
                 hd.copy_(od, non_blocking=True)
                 hi.copy_(oi, non_blocking=True)
-                host_d.append(hd)
-                host_i.append(hi)
+                groups.append((rs, hd, hi))
             ev = torch.cuda.Event()
             ev.record(stream)
-        return _Launch(stream, ev, list(reqs), host_d, host_i, dev, sum(self._bytes(r) for r in reqs))
+        return _Launch(stream, ev, groups, dev, sum(self._bytes(r) for r in reqs))
 
     def run(self, requests, time_limit: float = 600.0):
         """Serve `requests` (sorted by arrival); returns them with results and completion times."""
@@ -122,8 +124,13 @@ class Server:
             for L in list(inflight):
                 if L.event.query():
                     tdone = time.perf_counter() - t0
-                    for r, hd, hi in zip(L.reqs, L.host_d, L.host_i):
-                        r.out_d, r.out_i, r.done = hd.numpy(), hi.numpy(), tdone
+                    for rs, hd, hi in L.groups:
+                        d_all, i_all = hd.numpy(), hi.numpy()
+                        off = 0
+                        for r in rs:
+                            m = r.queries.shape[0]
+                            r.out_d, r.out_i, r.done = d_all[off:off + m], i_all[off:off + m], tdone
+                            off += m
                     inflight.remove(L)
                     free.append(L.stream)
                     used -= L.nbytes

@@ -62,7 +62,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t CAND_CAP = 64;  // candidates per query (one per lane of the recheck)
 constexpr int KP = 16;             // largest k on this path
-constexpr uint32_t MAX_SLICES = 32;
+constexpr int LISTS_PER_LANE = 4;  // k_screen_select reads 64 x 4 per-lane lists at most
+constexpr uint32_t MAX_SLICES = 32 * LISTS_PER_LANE;  // two lists (lane halves) per slice
 
 struct SArgs {
   const float* __restrict__ rows32;  // [n_pad][2 KS], tree order, zero-padded
@@ -107,20 +108,23 @@ __global__ __launch_bounds__(64) void k_screen(SArgs a) {
   }
   float thr = INFINITY;  // la[KP - 1]
 
+  // Point tiles are loaded two tiles ahead of the contraction (rotating
+  // register buffers); the tile's |p~|^2 are loaded as its contraction starts.
   constexpr int NV = KS / 4;
-  float4 cur[NV], nxt[NV];
-  float4 pn_cur[4], pn_nxt[4];
-  auto load = [&](uint32_t t, float4* dst, float4* pn) {
+  float4 cur[NV], nx1[NV], nx2[NV];
+  auto load = [&](uint32_t t, float4* dst) {
     const float4* src = reinterpret_cast<const float4*>(a.rows32 + ((size_t)t * 32 + j) * (2 * KS) + h * KS);
 #pragma unroll
     for (int v = 0; v < NV; ++v) dst[v] = src[v];
+  };
+  if (t0 < t1) load(t0, cur);
+  if (t0 + 1 < t1) load(t0 + 1, nx1);
+  for (uint32_t t = t0; t < t1; ++t) {
+    if (t + 2 < t1) load(t + 2, nx2);
+    float4 pn_cur[4];
     const float4* ps = reinterpret_cast<const float4*>(a.pnorm + (size_t)t * 32 + 4 * h);
 #pragma unroll
-    for (int g = 0; g < 4; ++g) pn[g] = ps[2 * g];
-  };
-  if (t0 < t1) load(t0, cur, pn_cur);
-  for (uint32_t t = t0; t < t1; ++t) {
-    if (t + 1 < t1) load(t + 1, nxt, pn_nxt);
+    for (int g = 0; g < 4; ++g) pn_cur[g] = ps[2 * g];
     floatx16 acc = {};
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -151,9 +155,10 @@ __global__ __launch_bounds__(64) void k_screen(SArgs a) {
       }
     }
 #pragma unroll
-    for (int v = 0; v < NV; ++v) cur[v] = nxt[v];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) pn_cur[g] = pn_nxt[g];
+    for (int v = 0; v < NV; ++v) {
+      cur[v] = nx1[v];
+      nx1[v] = nx2[v];
+    }
   }
   if (qvalid) {
     const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
@@ -177,27 +182,44 @@ struct SelArgs {
   double pmax, cutoff, max_abs;
 };
 
-// One wave per query, lane l reading list l: k rounds of wave-min over the
-// list heads give A_k, then tau_A from the error bound, the completeness
-// check and the candidates (list entries with A <= tau_A, at most CAND_CAP).
+// One wave per query; lane l reads lists l, l + 64, ... (at most LISTS_PER_LANE):
+// k rounds of wave-min over the list heads give A_k, then tau_A from the error
+// bound, the completeness check and the candidates (list entries with
+// A <= tau_A, at most CAND_CAP).
 __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
-  const bool mine = lane < a.lists;
-  const float* la = a.part_a + (qi * a.lists + lane) * KP;
-  const uint32_t* li = a.part_i + (qi * a.lists + lane) * KP;
-  uint32_t head = 0;
-  float cur = mine ? la[0] : INFINITY;
+  const float* la0 = a.part_a + qi * a.lists * KP;
+  const uint32_t* li0 = a.part_i + qi * a.lists * KP;
+  uint32_t head[LISTS_PER_LANE];
+  float hv[LISTS_PER_LANE];  // value at each head (+inf when exhausted or absent)
+#pragma unroll
+  for (int c = 0; c < LISTS_PER_LANE; ++c) {
+    const uint32_t L = lane + 64 * c;
+    head[c] = 0;
+    hv[c] = L < a.lists ? la0[L * KP] : INFINITY;
+  }
   float ak = INFINITY;
   for (uint32_t r = 0; r < a.k; ++r) {
+    float cur = hv[0];
+#pragma unroll
+    for (int c = 1; c < LISTS_PER_LANE; ++c) cur = fminf(cur, hv[c]);
     float m = cur;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o));
     ak = m;
-    const uint64_t hit = __ballot(mine && cur == m);
-    if (hit && lane == (uint32_t)(__ffsll((unsigned long long)hit) - 1)) {
-      ++head;
-      cur = head < KP ? la[head] : INFINITY;
+    if (m == INFINITY) break;
+    const uint64_t hit = __ballot(cur == m);
+    if (lane == (uint32_t)(__ffsll((unsigned long long)hit) - 1)) {
+      bool done = false;
+#pragma unroll
+      for (int c = 0; c < LISTS_PER_LANE; ++c) {
+        if (!done && hv[c] == m) {
+          done = true;
+          ++head[c];
+          hv[c] = head[c] < KP ? la0[(lane + 64 * c) * KP + head[c]] : INFINITY;
+        }
+      }
     }
   }
   double qn = 0.0, qmax = 0.0;
@@ -233,18 +255,27 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
     ta = (float)t;
     if ((double)ta < t) ta = nextafterf(ta, INFINITY);
   }
-  // Completeness: a full list whose last value is <= tau_A may have dropped
-  // a candidate.
+  // Completeness: a full list whose last value is <= tau_A may have dropped a
+  // candidate.  Unfilled slots (A = +inf, position 0xFFFFFFFF) end a list.
+  uint32_t cnt_c[LISTS_PER_LANE];
   uint32_t mcount = 0;
   bool complete = true;
-  if (mine) {
-    const float last = la[KP - 1];
-    complete = last == INFINITY || last > ta;
-    // Unfilled slots (A = +inf, position 0xFFFFFFFF) end the sorted list.
-    while (mcount < KP && li[mcount] != 0xFFFFFFFFu && la[mcount] <= ta) ++mcount;
+#pragma unroll
+  for (int c = 0; c < LISTS_PER_LANE; ++c) {
+    const uint32_t L = lane + 64 * c;
+    uint32_t n = 0;
+    if (L < a.lists) {
+      const float* la = la0 + L * KP;
+      const uint32_t* li = li0 + L * KP;
+      const float last = la[KP - 1];
+      complete = complete && (last == INFINITY || last > ta);
+      while (n < KP && li[n] != 0xFFFFFFFFu && la[n] <= ta) ++n;
+    }
+    cnt_c[c] = n;
+    mcount += n;
   }
   const bool ok = qmax <= 1e18 && a.max_abs <= 1e18 && !__ballot(!complete);
-  // Exclusive prefix sum of the per-list counts.
+  // Exclusive prefix sum of the per-lane counts.
   uint32_t incl = mcount;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -254,8 +285,13 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   const uint32_t total = __shfl(incl, 63);
   const bool good = ok && total <= CAND_CAP;
   if (good) {
-    const uint32_t off = incl - mcount;
-    for (uint32_t e = 0; e < mcount; ++e) a.cand[qi * CAND_CAP + off + e] = li[e];
+    uint32_t off = incl - mcount;
+#pragma unroll
+    for (int c = 0; c < LISTS_PER_LANE; ++c) {
+      const uint32_t* li = li0 + (lane + 64 * c) * KP;
+      for (uint32_t e = 0; e < cnt_c[c]; ++e) a.cand[qi * CAND_CAP + off + e] = li[e];
+      off += cnt_c[c];
+    }
   }
   if (lane == 0) {
     a.cnt[qi] = good ? total : 0;

@@ -58,10 +58,12 @@ constexpr int STK_MAX = 64;
 
 // Waves per SIMD the register-list kernels are compiled for.  The descent is
 // bound by the latency of its dependent node loads more than by issue, so
-// occupancy pays: rows of 1..4 coordinates run 8 waves (a few registers of
-// the query start-up spill), wider rows 6 (they spill in their hot loops at 8).
+// occupancy pays as long as the hot loops keep their registers: rows of 1..4
+// coordinates run 8 waves (a few registers of the query start-up spill), rows
+// of 8 run 6 (7 or 8 spill in the hot loops), rows of 16 take the registers
+// they need (120: 4 waves; at 5 or 6 waves their leaf loop spills).
 template <int DT>
-constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : 6; }
+constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : DT == 8 ? 6 : 1; }
 // Hits in one leaf chunk from which a group inserts them with one bitonic
 // merge instead of one by one (the merge costs about ten single inserts).
 constexpr uint32_t BULK_MIN = 10;

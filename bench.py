@@ -265,7 +265,8 @@ def run_c3(args):
 def run_c4(args):
     """256 trees (n log-uniform in [1e3,1e6], d in {2,3,8,16}, float32 uniform
     promoted to float64), per-tree query counts log-uniform in [1, 1e5], k in
-    {1, 8, 32}; leafsize autotuned over {8,16,32,64}.  One persistent launch
+    {1, 8, 32}; leafsize autotuned over {8,16,32,64} and lanes per query over
+    the widths that hold k.  One persistent launch
     versus one launch per tree versus the per-tree launches as a HIP graph."""
     import torch
 
@@ -295,16 +296,17 @@ def run_c4(args):
         od, oi = OracleTree(data, ls).query(queries, k=k)
         gd, gi = batch.results(j)
         parity &= bool(np.array_equal(gi.cpu().numpy(), oi) and gd.cpu().numpy().tobytes() == od.tobytes())
-    leafs = {}
+    leafs, groups = {}, {}
     for tree, _, _ in items:
         leafs[tree.leafsize] = leafs.get(tree.leafsize, 0) + 1
+        groups[tree.group] = groups.get(tree.group, 0) + 1
     out = {
         "metric": "k-NN queries/s over a 256-tree batch",
         "config": {"workload": "knn-256-trees-n1e3..1e6-d{2,3,8,16}-k{1,8,32}", "n_queries": nq,
                    "n_points": int(sum(s[0] for s in specs))},
         "queries_per_s": res,
         "persistent_vs_per_tree": res["persistent"] / res["per_tree"],
-        "autotuned_leafsize_counts": leafs, "autotune_s": tune_s,
+        "autotuned_leafsize_counts": leafs, "autotuned_lanes_per_query_counts": groups, "autotune_s": tune_s,
         "parity_first_8_trees_bit_exact": parity,
     }
     for tree, _, _ in items:

@@ -67,6 +67,11 @@ enum kdt_error {
                                         screen also for 16 <= ndim < 32, where the
                                         traversal still prunes part of the tree (a
                                         per-tree choice, made by timing both) */
+#define KDT_QUERY_GROUP_SHIFT 4u /* bits 4-6: lanes per query in the traversal, 0 = chosen
+                                    from k (16 for k <= 16, 32 for k <= 32, else 64),
+                                    1 = 8, 2 = 16, 3 = 32, 4 = 64 lanes (ignored when fewer
+                                    than k); a per-tree choice, made by timing (kdtree.forest) */
+#define KDT_QUERY_GROUP_MASK 0x70u
 #define KDT_QUERY_SCREEN_ONLY 0x4u /* diagnostic, with KDT_QUERY_FP32_SCREEN on the
                                       matrix-core path: skip the traversal of the
                                       queries the certificate does not cover, whose
@@ -87,6 +92,8 @@ typedef struct kdt_batch_item {
   uint64_t k;
   double* out_dist;
   int64_t* out_idx;
+  uint32_t group;  /* lanes per query for this item's tree (8, 16, 32, 64; 0 = from the
+                      flags or from k), e.g. the width an autotuner picked */
 } kdt_batch_item;
 
 /* Node record exported for inspection (24 bytes, the reference's Node budget,

@@ -69,6 +69,7 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   qp.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   qp.screen_only = (flags & KDT_QUERY_SCREEN_ONLY) != 0;
   qp.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
+  qp.group = kdt::requested_group(flags);
   std::string smsg;
   const int scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
   if (q_on_device && out_on_device && (flags & KDT_QUERY_ASSUME_FINITE)) {

@@ -159,6 +159,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   if (rc) return batch_fail(rc, msg);
   base.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   base.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
+  base.group = requested_group(flags);
   const bool per_tree = (flags & KDT_BATCH_PER_TREE_LAUNCHES) != 0;
   const bool no_order = (flags & KDT_BATCH_NO_ORDER) != 0;
 
@@ -194,6 +195,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     qp.k = it.k;
     qp.nq = it.n_queries;
     qp.allow_order = !no_order;
+    if (it.group) qp.group = (int)it.group;
     rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, s, msg);
     if (rc) return batch_fail(rc, msg);
   }
@@ -201,7 +203,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   // Shared launches: items numbered class by class, so each class is one
   // contiguous range of the batch's query numbers (and of the sorted order).
   std::vector<BItem> h;
-  uint64_t first[6] = {0}, count[6] = {0}, kmax[6] = {0};
+  uint64_t first[6] = {0}, count[6] = {0}, kmax[6] = {0}, gmax[6] = {0};
   uint32_t stk[6] = {2, 2, 2, 2, 2, 2};
   uint64_t total = 0;
   for (int c = 0; c < 6; ++c) {
@@ -216,6 +218,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
       b.q_begin = total;
       total += it.n_queries;
       kmax[c] = std::max<uint64_t>(kmax[c], it.k);
+      gmax[c] = std::max<uint64_t>(gmax[c], it.group ? it.group : (uint64_t)base.group);
       stk[c] = std::max<uint32_t>(stk[c], b.a.stk);
       h.push_back(b);
     }
@@ -264,7 +267,10 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     const uint32_t n = (uint32_t)h.size();
     for (int c = 0; c < 6; ++c) {
       if (!count[c]) continue;
-      const int G = kmax[c] <= 16 ? 16 : kmax[c] <= 32 ? 32 : 64;  // G = 8 measured no faster
+      // Lanes per query: the widest requested by the class's items (an
+      // autotuner's per-tree choice) when it holds every item's k, else from k.
+      int G = kmax[c] <= 16 ? 16 : kmax[c] <= 32 ? 32 : 64;
+      if ((gmax[c] == 8 || gmax[c] == 16 || gmax[c] == 32 || gmax[c] == 64) && gmax[c] >= kmax[c]) G = (int)gmax[c];
       switch (base.metric) {
         case M_L1:
           launch_class_width<M_L1>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);

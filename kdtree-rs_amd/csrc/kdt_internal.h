@@ -66,8 +66,15 @@ struct QueryParams {
   bool fp32_screen = false;
   bool screen_only = false;    // diagnostic: no traversal after the matrix-core screen
   bool matrix_screen = false;  // matrix-core screen also for 16 <= ndim < 32
+  int group = 0;               // lanes per query requested (8, 16, 32, 64), 0 = from k
   bool allow_order = true;  // large batches may run in leaf order
 };
+
+// Lanes per query requested by KDT_QUERY_GROUP bits of the query flags (0 = none).
+inline int requested_group(uint32_t flags) {
+  const uint32_t g = (flags >> 4) & 7u;
+  return g >= 1 && g <= 4 ? (4 << g) : 0;
+}
 
 // Metric, cutoff and eps factor from (p, max_distance, eps) with the checks
 // and messages of src/query.rs:49-62 and src/metric.rs:45-71; 0 or an error code.

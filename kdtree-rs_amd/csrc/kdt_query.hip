@@ -26,14 +26,12 @@ bool compiled_width(uint64_t d) { return d == 1 || d == 2 || d == 3 || d == 4 ||
 // Lanes per query: the narrowest group that holds the k list.  Narrow groups
 // keep more queries in flight per wavefront, which measured faster than
 // covering a whole leaf per pass (a 32-row leaf takes two passes at G = 16).
-// KDT_GROUP_WIDTH (16, 32 or 64) overrides the choice when it still holds k.
-int group_width(uint64_t k, uint64_t leafsize) {
-  (void)leafsize;
+// A requested width (query flags, picked per tree by timing) wins when it
+// still holds k.
+int group_width(uint64_t k, int requested) {
   int g = k <= 16 ? 16 : k <= 32 ? 32 : 64;
-  if (const char* env = std::getenv("KDT_GROUP_WIDTH")) {
-    const int want = std::atoi(env);
-    if ((want == 8 || want == 16 || want == 32 || want == 64) && (uint64_t)want >= k) g = want;
-  }
+  if ((requested == 8 || requested == 16 || requested == 32 || requested == 64) && (uint64_t)requested >= k)
+    g = requested;
   return g;
 }
 }  // namespace
@@ -154,7 +152,7 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
   const bool reg_ok = qp.metric != M_LP && compiled_width(t.d) && qp.k <= 64;
   int rc = -1;
   if (reg_ok) {
-    const int G = group_width(qp.k, t.leafsize);
+    const int G = group_width(qp.k, qp.group);
     const bool screen = qp.fp32_screen && qp.metric == M_L2 && t.d <= 64;
     switch (qp.metric) {
       case M_L1: rc = launch_knn_l1(a, G, s); break;

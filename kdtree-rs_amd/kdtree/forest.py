@@ -25,6 +25,8 @@ from ._core import load_library
 
 KDT_QUERY_FP32_SCREEN = 0x1
 KDT_QUERY_ASSUME_FINITE = 0x2
+KDT_QUERY_MATRIX_SCREEN = 0x8
+KDT_QUERY_GROUP_SHIFT = 4
 KDT_BATCH_PER_TREE_LAUNCHES = 0x100
 KDT_BATCH_NO_ORDER = 0x200
 
@@ -38,6 +40,7 @@ class BatchItem(ctypes.Structure):
         ("k", ctypes.c_uint64),
         ("out_dist", ctypes.c_void_p),
         ("out_idx", ctypes.c_void_p),
+        ("group", ctypes.c_uint32),  # lanes per query, 0 = automatic
     ]
 
 
@@ -74,6 +77,7 @@ class DeviceTree:
     n: int
     d: int
     leafsize: int
+    group: int = 0  # lanes per query chosen by the autotuner (0 = automatic)
     tune_ms: dict = field(default_factory=dict)
 
     def free(self):
@@ -94,10 +98,18 @@ def build(data, leafsize: int, stream=None) -> DeviceTree:
     return DeviceTree(h, t.shape[0], t.shape[1], leafsize)
 
 
+def group_flags(group: int) -> int:
+    """Query-flag bits that request `group` lanes per query (0: automatic)."""
+    code = {0: 0, 8: 1, 16: 2, 32: 3, 64: 4}[group]
+    return code << KDT_QUERY_GROUP_SHIFT
+
+
 def query_device(tree: DeviceTree, q, k: int, out_d=None, out_i=None, flags: int = KDT_QUERY_ASSUME_FINITE,
                  stream=None, p: float = 2.0):
     torch = _torch()
     lib = _lib()
+    if tree.group and not flags & (7 << KDT_QUERY_GROUP_SHIFT):
+        flags |= group_flags(tree.group)
     nq = q.shape[0]
     if out_d is None:
         out_d = torch.empty((nq, k), dtype=torch.float64, device=q.device)
@@ -108,34 +120,43 @@ def query_device(tree: DeviceTree, q, k: int, out_d=None, out_i=None, flags: int
     return out_d, out_i
 
 
-def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), reps: int = 3) -> DeviceTree:
-    """Build with each leafsize, time `reps` queries of the sample, keep the fastest.
+def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), groups=None,
+                   reps: int = 3) -> DeviceTree:
+    """Build with each leafsize, time `reps` queries of the sample with each
+    group width (lanes per query that hold k), keep the fastest pair.
 
     The leafsize changes the tree, and for 4- and 8-dimensional rows the
     reference's summation order inside a leaf, so the chosen leafsize is part
-    of the result's definition: the oracle must use the same one.
+    of the result's definition: the oracle must use the same one.  The group
+    width changes only the speed.
     """
     torch = _torch()
     q = torch.from_numpy(np.ascontiguousarray(sample_queries, np.float64)).cuda()
     if not isinstance(data, torch.Tensor):
         data = torch.from_numpy(np.ascontiguousarray(data, np.float64)).cuda()
-    best, timings = None, {}
+    if groups is None:
+        groups = [g for g in (8, 16, 32, 64) if g >= k][:3]
+    best, best_ms, timings = None, None, {}
     for ls in candidates:
         tree = build(data, ls)
-        out_d, out_i = query_device(tree, q, k)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            query_device(tree, q, k, out_d, out_i)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / reps
-        timings[ls] = ms
-        if best is None or ms < timings[best.leafsize]:
-            if best is not None:
-                best.free()
-            best = tree
-        else:
+        keep = False
+        for g in groups:
+            tree.group = g
+            out_d, out_i = query_device(tree, q, k)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                query_device(tree, q, k, out_d, out_i)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+            timings[(ls, g)] = ms
+            if best_ms is None or ms < best_ms:
+                if best is not None and best is not tree:
+                    best.free()
+                best, best_ms, best_g, keep = tree, ms, g, True
+        if not keep:
             tree.free()
+    best.group = best_g
     best.tune_ms = timings
     return best
 
@@ -152,7 +173,7 @@ class Batch:
             out_i = torch.empty((q.shape[0], k), dtype=torch.int64, device=q.device)
             self.items.append((tree, q, k, out_d, out_i))
             self.arr[j] = BatchItem(tree.handle.value, q.data_ptr(), q.shape[0], tree.d, k, out_d.data_ptr(),
-                                    out_i.data_ptr())
+                                    out_i.data_ptr(), tree.group)
         self.graph = None
 
     @property

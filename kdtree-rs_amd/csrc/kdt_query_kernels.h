@@ -3,7 +3,7 @@
 // Replaces Tree::query / run / Descent (src/query.rs:29-494) and the leaf
 // scans, point and box distances of src/kernel.rs:167-616.
 //
-// Work split: a group of G lanes (G in {16, 32, 64}, a power of two dividing
+// Work split: a group of G lanes (G in {8, 16, 32, 64}, a power of two dividing
 // the 64-lane wavefront) owns one query at a time.  The group walks the tree
 // together -- every traversal value (node id, cell bound, k-th distance, stack
 // pointer) is the same in all G lanes, so a group never diverges internally --
@@ -22,7 +22,8 @@
 //   BOXNEAR  deferred near child of an order_by_box node: box gate, descend
 //   RESTORE  undo of the cell-bound axis set when a far child was entered
 // Every inner node on the current path holds at most one deferred entry and
-// one RESTORE; a path has at most 32 inner nodes (n < 2^32), so 64 entries.
+// one RESTORE; a path has at most 32 inner nodes (n < 2^32), so 64 entries
+// (32 for trees of depth <= 16).
 //
 // Because visits, gates and bound updates happen in the reference's order,
 // the result is the reference's for every eps, not only eps = 0.  Distances
@@ -720,11 +721,8 @@ template <int DT, int G, int M, bool SCREEN>
 inline void launch_reg(const KArgs& a, hipStream_t s) {
   constexpr int GPB = 256 / G;
   // One group per query (measured faster than a resident grid that hands
-  // each group several queries); KDT_QUERY_BLOCKS caps the grid for tuning.
-  uint64_t cap = 1u << 30;
-  if (const char* env = std::getenv("KDT_QUERY_BLOCKS")) cap = std::max<uint64_t>(1, std::strtoull(env, nullptr, 10));
-  uint64_t blocks = (a.nq + GPB - 1) / GPB;
-  if (blocks > cap) blocks = cap;
+  // each group several queries); a larger batch loops (knn_body's next()).
+  const uint64_t blocks = std::min<uint64_t>((a.nq + GPB - 1) / GPB, 1u << 30);
   if (a.stk <= 32) knn_reg_kernel<DT, G, M, SCREEN, 32><<<(uint32_t)blocks, 256, 0, s>>>(a);
   else knn_reg_kernel<DT, G, M, SCREEN, STK_MAX><<<(uint32_t)blocks, 256, 0, s>>>(a);
 }

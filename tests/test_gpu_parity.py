@@ -198,3 +198,25 @@ def test_query_ordering_is_invisible(monkeypatch, order):
     data = np.random.default_rng(120).uniform(size=(50_000, 3))
     queries = np.random.default_rng(121).uniform(size=(20_000, 3))
     _query_case(data, queries, k=16)
+
+
+@pytest.mark.parametrize("group", [8, 16, 32, 64])
+@pytest.mark.parametrize("dims,k", [(3, 1), (3, 8), (8, 8), (16, 5), (2, 16)])
+def test_lanes_per_query_do_not_change_results(group, dims, k):
+    # The group width (lanes per query, a per-tree autotuner choice) only
+    # changes speed; widths below k fall back to the automatic one.
+    import ctypes
+
+    data = np.random.default_rng(130 + dims).normal(size=(9000, dims))
+    queries = np.random.default_rng(140 + k).normal(size=(9000, dims))  # >= 8192: leaf-ordered path too
+    od, oi = OracleTree(data, 16).query(queries, k=k)
+    tree = KDTree(data, leafsize=16)
+    code = {8: 1, 16: 2, 32: 3, 64: 4}[group]
+    out_d = np.empty((queries.shape[0], k))
+    out_i = np.empty((queries.shape[0], k), dtype=np.int64)
+    rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(queries.ctypes.data), queries.shape[0], dims, k, 2.0,
+                             np.inf, 0.0, code << 4, ctypes.c_void_p(out_d.ctypes.data),
+                             ctypes.c_void_p(out_i.ctypes.data))
+    assert rc == 0, tree._lib.kdt_last_error().decode()
+    np.testing.assert_array_equal(out_i, oi)
+    assert out_d.tobytes() == od.tobytes()

@@ -677,19 +677,24 @@ __global__ void k_validate_nodes(const DevNode* __restrict__ nodes, uint64_t n_n
   if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(err, 8);
 }
 
-// Matrix-core screen rows: tree-order rows rounded to f32, zero-padded to
-// `stride` columns and n_pad rows; pnorm = |row32|^2 (+inf on padding rows);
-// *pmax_bits = max over rows of |row32| rounded up, as f32 bits.
-__global__ void k_rows32(const double* __restrict__ rows, uint64_t n, uint32_t D, uint32_t stride,
-                         uint64_t n_pad, float* __restrict__ rows32, float* __restrict__ pnorm,
-                         uint32_t* pmax_bits) {
+// Matrix-core screen rows: tree-order rows rounded to f32 (v), zero-padded to
+// `stride` columns and n_pad rows, each stored as two bf16 halves,
+// hi = bf16(v) and lo = bf16(v - hi): row r is [hi x stride | lo x stride] in
+// rows_bf.  pnorm = |v|^2 (+inf on padding rows); *pmax_bits = max over rows
+// of |v| rounded up, as f32 bits.
+__global__ void k_rows_split(const double* __restrict__ rows, uint64_t n, uint32_t D, uint32_t stride,
+                             uint64_t n_pad, uint16_t* __restrict__ rows_bf, float* __restrict__ pnorm,
+                             uint32_t* pmax_bits) {
   float mx = 0.0f;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_pad;
        r += (uint64_t)gridDim.x * blockDim.x) {
     double s = 0.0;
     for (uint32_t c = 0; c < stride; ++c) {
       const float v = (r < n && c < D) ? (float)rows[r * D + c] : 0.0f;
-      rows32[r * stride + c] = v;
+      const __bf16 hi = (__bf16)v;
+      const __bf16 lo = (__bf16)(v - (float)hi);
+      rows_bf[r * 2 * stride + c] = __builtin_bit_cast(uint16_t, hi);
+      rows_bf[r * 2 * stride + stride + c] = __builtin_bit_cast(uint16_t, lo);
       s += (double)v * (double)v;
     }
     pnorm[r] = r < n ? (float)s : INFINITY;
@@ -812,7 +817,8 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<LvlNode> d_lvl;
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
-  DevBuf<float> pts32, rows32, pnorm;
+  DevBuf<float> pts32, pnorm;
+  DevBuf<uint16_t> rows_bf;
   DevBuf<uint32_t> idx, tidx, alist, blist, cnt, off, hist, node_len, d_leaves;
   DevBuf<uint64_t> ukey;
   DevBuf<Sel> sel;
@@ -830,7 +836,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   const uint32_t stride32 = D <= 16 ? 16 : D <= 32 ? 32 : 64;
   const uint64_t n_pad = (n + 31) / 32 * 32;
   if (wide) {
-    KDT_HIP_CHECK(rows32.alloc(n_pad * stride32));
+    KDT_HIP_CHECK(rows_bf.alloc(n_pad * 2 * stride32));
     KDT_HIP_CHECK(pnorm.alloc(n_pad));
   }
   KDT_HIP_CHECK(idx.alloc(n));
@@ -919,8 +925,8 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
                                                                     node_len.p);
   k_leaf_buckets<<<(uint32_t)(leaves.size() / 2), 64, 0, s>>>(rows.p, Du, d_leaves.p, pts.p, pts32.p);
   if (wide) {
-    k_rows32<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
-        rows.p, n, Du, stride32, n_pad, rows32.p, pnorm.p, reinterpret_cast<uint32_t*>(err.p) + 2);
+    k_rows_split<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
+        rows.p, n, Du, stride32, n_pad, rows_bf.p, pnorm.p, reinterpret_cast<uint32_t*>(err.p) + 2);
   }
   k_validate_nodes<<<(uint32_t)std::min<uint64_t>((t.n_nodes + 255) / 256, 4096), 256, 0, s>>>(
       nodes.p, t.n_nodes, n, Du, err.p);
@@ -957,7 +963,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
     float pm;
     std::memcpy(&pm, &h_pmax, 4);
     t.pmax = (double)pm * (1.0 + 1e-6);
-    t.rows32 = rows32.release();
+    t.rows_bf = rows_bf.release();
     t.pnorm = pnorm.release();
   }
   return 0;

@@ -9,11 +9,12 @@
 //   idx    u32[n_points]               original row of tree position
 //   rows   f64[n_points][ndim]         rows in tree order (row-major), the
 //                                      reference's Tree::data
-//   rows32 f32[n_pad][stride]          16 <= ndim <= 64 only: rows in tree order
-//                                      rounded to f32, zero-padded to stride 16,
-//                                      32 or 64 and to n_pad = 32 * ceil(n / 32)
-//                                      rows, for the matrix-core screen
-//   pnorm  f32[n_pad]                  |row32|^2 (+inf on padding rows)
+//   rows_bf bf16[n_pad][2][stride]     16 <= ndim <= 64 only: rows in tree order
+//                                      rounded to f32 (v), zero-padded to stride
+//                                      16, 32 or 64 and to n_pad = 32 * ceil(n / 32)
+//                                      rows, as bf16 hi = bf16(v) then lo =
+//                                      bf16(v - hi), for the matrix-core screen
+//   pnorm  f32[n_pad]                  |v|^2 (+inf on padding rows)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,7 +51,7 @@ struct Tree {
   uint32_t* idx = nullptr;
   double* rows = nullptr;
   double max_abs = 0.0;  // max |coordinate| over the data (root box)
-  float* rows32 = nullptr;
+  uint16_t* rows_bf = nullptr;  // bf16 bit patterns
   float* pnorm = nullptr;
   double pmax = 0.0;     // upper bound on max |row32|
 };

@@ -10,8 +10,9 @@
 // contraction, which is what the matrix cores do.
 //
 // Steps for a query matrix against one tree:
-//   1. k_screen: for a tile of 32 queries and a slice of the points,
-//      v_mfma_f32_32x32x2f32 forms q.p in FP32 and each lane keeps the KP = 16
+//   1. k_screen: for a tile of 32 queries and a slice of the points, the
+//      matrix cores form q.p from bf16 halves of the FP32-rounded rows (three
+//      v_mfma_f32_32x32x16_bf16 per 16 coordinates) and each lane keeps the KP = 16
 //      smallest  A = |q|^2 + |p|^2 - 2 q.p  of the points it sees, with the
 //      points' tree positions;
 //   2. k_screen_select: the k-th smallest A over all of a query's lists gives
@@ -32,11 +33,15 @@
 //      this top-k.  Queries whose certificate fails are flagged;
 //   4. the traversal kernels run the flagged queries (KArgs::only).
 //
-// FP32 error bound (u = 2^-24, q~ and p~ the rows rounded to FP32, P the
-// largest |p~| of the tree, N = |q~| + P):
-//   |A - |q~ - p~|^2| <= E2 = (2 g + 8u) N^2 + D 2^-125,  g = (2D+4)u/(1-(2D+4)u)
-//     (MFMA accumulation of D products, FP32 rounding of |q~|^2, |p~|^2, the
-//      sum and the final fma; flushed subnormal products);
+// Error bound (u = 2^-24, q~ and p~ the rows rounded to FP32, P the largest
+// |p~| of the tree, N = |q~| + P).  Splitting an FP32 x into bf16 halves
+// (precision 8 bits) leaves |x - xh - xl| <= 2^-16 |x| and |xl| <= 2^-8 |x|
+// (1 + 2^-8), so the three products miss at most 3.1 2^-16 |x| |y| per
+// coordinate; the matrix core adds the 3 D exact products in FP32:
+//   |A - |q~ - p~|^2| <= E2 = (2 g + 8u) N^2 + D 2^-110 (1 + N)^2,
+//     g = 3.1 2^-16 + 1.03 (3D+4)u / (1 - (3D+4)u)
+//     (the dot product, FP32 rounding of |q~|^2, |p~|^2, the sum and the final
+//      fma; flushed subnormal halves and products);
 //   | |q~ - p~| - |q - p| | <= e1 = u (1 + 2u) N + sqrt(D) 2^-148;
 //   the FP64 reduced distance d of the reference satisfies d / s <= |q-p|^2
 //     <= d s with s = 1 + 2 (D + 3) 2^-53.
@@ -59,6 +64,7 @@ int launch_knn_lds(const KArgs& a, int metric, hipStream_t s);
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr uint32_t CAND_CAP = 64;  // candidates per query (one per lane of the recheck)
 constexpr int KP = 16;             // largest k on this path
@@ -66,7 +72,7 @@ constexpr int LISTS_PER_LANE = 4;  // k_screen_select reads 64 x 4 per-lane list
 constexpr uint32_t MAX_SLICES = 32 * LISTS_PER_LANE;  // two lists (lane halves) per slice
 
 struct SArgs {
-  const float* __restrict__ rows32;  // [n_pad][2 KS], tree order, zero-padded
+  const uint16_t* __restrict__ rows_bf;  // [n_pad][hi 2 KS | lo 2 KS] bf16, tree order, zero-padded
   const float* __restrict__ pnorm;   // [n_pad] |p~|^2, +inf on padding rows
   const double* __restrict__ q;      // [nq][D]
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
@@ -75,68 +81,92 @@ struct SArgs {
   uint32_t D, n, tiles_per_slice, n_tiles, slices;
 };
 
-// One wave: 32 queries (B operand, columns) against 32-row tiles of points
-// (A operand, rows).  Lane l holds query j = l & 31, coordinates
-// [h KS, h KS + KS) with h = l >> 5, in both operands, so instruction s
-// contracts coordinates s and KS + s.  The accumulator holds D[i][j] for
-// point rows i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
-template <int KS>
-__global__ __launch_bounds__(64) void k_screen(SArgs a) {
-  const uint32_t lane = threadIdx.x;
-  const uint32_t j = lane & 31, h = lane >> 5;
-  const uint64_t qrow = (uint64_t)blockIdx.x * 32 + j;
-  const bool qvalid = qrow < a.nq;
-  float qf[KS];
-  double qn_part = 0.0;
-#pragma unroll
-  for (int s = 0; s < KS; ++s) {
-    const uint32_t c = h * KS + s;
-    const float v = (qvalid && c < a.D) ? (float)a.q[qrow * a.D + c] : 0.0f;
-    qf[s] = v;
-    qn_part += (double)v * (double)v;
-  }
-  const float qn = (float)(qn_part + __shfl_xor(qn_part, 32));
-  const uint32_t t0 = blockIdx.y * a.tiles_per_slice;
-  const uint32_t t1 = min(a.n_tiles, t0 + a.tiles_per_slice);
+#ifndef KDT_SCREEN_WAVES
+#define KDT_SCREEN_WAVES 8
+#endif
+constexpr int SW = KDT_SCREEN_WAVES;  // waves (32-query tiles) per workgroup sharing each point tile
 
+// Point tile t, 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS], KS / 2 uint4),
+// staged global -> registers -> LDS by the whole workgroup: thread u copies
+// uint4 u, u + 64 SW, ... of the tile (coalesced), and threads 0..31 its |p~|^2.
+// (Nothing is read past the tile: TU and 32 bound every index.)
+template <int KS>
+struct Stage {
+  static constexpr int RU = KS / 2;                          // uint4 per row
+  static constexpr int TU = 32 * RU;                         // uint4 per tile
+  static constexpr int PT = (TU + 64 * SW - 1) / (64 * SW);  // uint4 per thread
+  static_assert(PT <= 2, "two staging registers per thread (SW >= 4)");
+};
+
+// The stage is three plain registers (v0, v1, pn), not an aggregate: an
+// aggregate carried through the loop ends up on the stack.  Loads are
+// unconditional, at clamped (in-bounds) indices; stage_store keeps u < TU.
+template <int KS>
+__device__ __forceinline__ void stage_load(const SArgs& a, uint32_t t, uint32_t tid, uint4& v0, uint4& v1,
+                                           float& pn) {
+  const uint4* src = reinterpret_cast<const uint4*>(a.rows_bf) + (size_t)t * Stage<KS>::TU;
+  v0 = src[min(tid, (uint32_t)Stage<KS>::TU - 1)];
+  if (Stage<KS>::PT > 1) v1 = src[min(tid + 64 * SW, (uint32_t)Stage<KS>::TU - 1)];
+  pn = a.pnorm[(size_t)t * 32 + (tid & 31)];
+}
+
+// LDS rows are padded by one uint4 (stride KS / 2 + 1 uint4), so the 32 lanes
+// of a half-wave reading 32 different rows hit distinct banks.
+template <int KS>
+__device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, float pnv, uint32_t tid, uint4* rows,
+                                            float* pn) {
+  constexpr int RU = Stage<KS>::RU;
+  if (tid < (uint32_t)Stage<KS>::TU) rows[(tid / RU) * (RU + 1) + tid % RU] = v0;
+  const uint32_t u = tid + 64 * SW;
+  if (Stage<KS>::PT > 1 && u < (uint32_t)Stage<KS>::TU) rows[(u / RU) * (RU + 1) + u % RU] = v1;
+  if (tid < 32) pn[tid] = pnv;
+}
+
+// One wave: 32 queries (B operand, columns) against 32-row tiles of points
+// (A operand, rows); the SW waves of a workgroup take SW consecutive query
+// tiles and share each point tile through LDS (one HBM read per workgroup
+// instead of one per wave).  Lane l holds query j = l & 31, coordinates
+// [h KS, h KS + KS) with h = l >> 5, in both operands; instruction s of a
+// product contracts coordinates 8 s .. 8 s + 7 of each half.  Each FP32 value
+// v is split into bf16 halves hi + lo (lo = bf16(v - hi)) and q.p is formed
+// by three v_mfma_f32_32x32x16_bf16 per 16 coordinates: qh.ph + ql.ph + qh.pl
+// (bf16 x bf16 products are exact in FP32; the dropped ql.pl and the split
+// residual are in the error bound).  The accumulator holds D[i][j] for point
+// rows i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
+template <int KS>
+struct Screen {
+  static constexpr int NS = KS / 8;  // bf16x8 operands per lane half
+  static constexpr int RU = KS / 2, RS = RU + 1;
+  bf16x8 qh[NS], ql[NS];
+  float qn;
   float la[KP];     // the KP smallest A seen by this lane, ascending
   uint32_t li[KP];  // their tree positions
-#pragma unroll
-  for (int e = 0; e < KP; ++e) {
-    la[e] = INFINITY;
-    li[e] = 0xFFFFFFFFu;
-  }
-  float thr = INFINITY;  // la[KP - 1]
+  float thr;        // la[KP - 1]
 
-  // Point tiles are loaded two tiles ahead of the contraction (rotating
-  // register buffers); the tile's |p~|^2 are loaded as its contraction starts.
-  constexpr int NV = KS / 4;
-  float4 cur[NV], nx1[NV], nx2[NV];
-  auto load = [&](uint32_t t, float4* dst) {
-    const float4* src = reinterpret_cast<const float4*>(a.rows32 + ((size_t)t * 32 + j) * (2 * KS) + h * KS);
+  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const float* pns) {
+    uint4 op[2 * NS];
+    const uint4* rb = rows + j * RS + h * NS;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) dst[v] = src[v];
-  };
-  if (t0 < t1) load(t0, cur);
-  if (t0 + 1 < t1) load(t0 + 1, nx1);
-  for (uint32_t t = t0; t < t1; ++t) {
-    if (t + 2 < t1) load(t + 2, nx2);
-    float4 pn_cur[4];
-    const float4* ps = reinterpret_cast<const float4*>(a.pnorm + (size_t)t * 32 + 4 * h);
+    for (int v = 0; v < NS; ++v) {
+      op[v] = rb[v];
+      op[NS + v] = rb[2 * NS + v];
+    }
+    float4 pg[4];
 #pragma unroll
-    for (int g = 0; g < 4; ++g) pn_cur[g] = ps[2 * g];
+    for (int g = 0; g < 4; ++g) pg[g] = reinterpret_cast<const float4*>(pns)[2 * g + h];
     floatx16 acc = {};
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].x, qf[4 * v + 0], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].y, qf[4 * v + 1], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].z, qf[4 * v + 2], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cur[v].w, qf[4 * v + 3], acc, 0, 0, 0);
+    for (int v = 0; v < NS; ++v) {
+      const bf16x8 ph = __builtin_bit_cast(bf16x8, op[v]);
+      const bf16x8 pl = __builtin_bit_cast(bf16x8, op[NS + v]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, qh[v], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, ql[v], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pl, qh[v], acc, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const float4 pg = pn_cur[r / 4];
-      const float pn = (r % 4 == 0) ? pg.x : (r % 4 == 1) ? pg.y : (r % 4 == 2) ? pg.z : pg.w;
+      const float4 g4 = pg[r / 4];
+      const float pn = (r % 4 == 0) ? g4.x : (r % 4 == 1) ? g4.y : (r % 4 == 2) ? g4.z : g4.w;
       const float A = fmaf(-2.0f, acc[r], qn + pn);  // +inf on padding rows
       if (A < thr) {  // insert (A, row), dropping the largest
         const uint32_t row = t * 32 + 8 * (r / 4) + 4 * h + (r % 4);
@@ -154,18 +184,82 @@ __global__ __launch_bounds__(64) void k_screen(SArgs a) {
         thr = la[KP - 1];
       }
     }
+  }
+};
+
+template <int KS>
+__global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
+  constexpr int RS = KS / 2 + 1;
+  __shared__ uint4 s_rows[2][32 * RS];
+  __shared__ __align__(16) float s_pn[2][32];
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const uint32_t j = lane & 31, h = lane >> 5;
+  const uint64_t qtile = (uint64_t)blockIdx.x * SW + (tid >> 6);
+  const bool active = qtile * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
+  const uint64_t qrow = qtile * 32 + j;
+  const bool qvalid = qrow < a.nq;
+  Screen<KS> sc;
+  double qn_part = 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      cur[v] = nx1[v];
-      nx1[v] = nx2[v];
+  for (int s = 0; s < KS; ++s) {
+    // Loads at clamped (in-bounds) indices, all issued before any is used;
+    // coordinates past D and rows past nq are zero.
+    const uint32_t c = h * KS + s;
+    const double raw = a.q[min(qrow, a.nq - 1) * a.D + min(c, a.D - 1)];
+    const float v = (qvalid && c < a.D) ? (float)raw : 0.0f;
+    const __bf16 hi = (__bf16)v;
+    sc.qh[s / 8][s % 8] = hi;
+    sc.ql[s / 8][s % 8] = (__bf16)(v - (float)hi);
+    qn_part += (double)v * (double)v;
+  }
+  sc.qn = (float)(qn_part + __shfl_xor(qn_part, 32));
+#pragma unroll
+  for (int e = 0; e < KP; ++e) {
+    sc.la[e] = INFINITY;
+    sc.li[e] = 0xFFFFFFFFu;
+  }
+  sc.thr = INFINITY;
+
+  const uint32_t t0 = blockIdx.y * a.tiles_per_slice;
+  const uint32_t t1 = min(a.n_tiles, t0 + a.tiles_per_slice);
+  // Tile t sits in LDS stage (t - t0) & 1 while it is contracted; tile t + 1
+  // is written to the other stage in the same iteration (its readers, tile
+  // t - 1, finished before the barrier that ended the previous iteration) and
+  // tile t + 2 is in flight to registers.  x and y alternate as the register
+  // stage of tiles t + 1 and t + 2.  Every path issues the same loads (tile
+  // indices clamped to t1 - 1, a harmless re-read at the end of the slice), so
+  // the compiler's wait before each LDS store covers only the tile it stores
+  // and the next tile's loads stay in flight.
+  // (A macro rather than a lambda: the two register stages must stay named
+  // registers, never a stack object behind a reference.)
+#define KDT_SCREEN_STEP(T, B, N0, N1, NP)                                                 \
+  do {                                                                                    \
+    if ((T) + 1 < t1) stage_store<KS>(N0, N1, NP, tid, s_rows[(B) ^ 1], s_pn[(B) ^ 1]); \
+    stage_load<KS>(a, min((T) + 3, t1 - 1), tid, N0, N1, NP);                           \
+    if (active && (T) < t1) sc.tile((T), j, h, s_rows[B], s_pn[B]);                       \
+    __syncthreads();                                                                      \
+  } while (0)
+  if (t0 < t1) {  // uniform over the workgroup
+    uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
+    float xp = 0.0f, yp = 0.0f;
+    stage_load<KS>(a, t0, tid, x0, x1, xp);
+    stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
+    stage_load<KS>(a, min(t0 + 1, t1 - 1), tid, x0, x1, xp);
+    __builtin_amdgcn_sched_barrier(0);  // keep the issue order of the loop: x's loads, then y's
+    stage_load<KS>(a, min(t0 + 2, t1 - 1), tid, y0, y1, yp);
+    __syncthreads();
+    for (uint32_t t = t0; t < t1; t += 2) {
+      KDT_SCREEN_STEP(t, 0, x0, x1, xp);
+      KDT_SCREEN_STEP(t + 1, 1, y0, y1, yp);
     }
   }
+#undef KDT_SCREEN_STEP
   if (qvalid) {
     const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
 #pragma unroll
     for (int e = 0; e < KP; ++e) {
-      a.part_a[base + e] = la[e];
-      a.part_i[base + e] = li[e];
+      a.part_a[base + e] = sc.la[e];
+      a.part_i[base + e] = sc.li[e];
     }
   }
 }
@@ -237,10 +331,10 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   // tau_A (every lane computes the same value)
   const double u = 0x1p-24;
   const double D = (double)a.D;
-  const double m2 = (2.0 * D + 4.0) * u;
-  const double g = m2 / (1.0 - m2);
+  const double m3 = (3.0 * D + 4.0) * u;
+  const double g = 3.1 * 0x1p-16 + 1.03 * m3 / (1.0 - m3);  // split residual + FP32 accumulation
   const double N = (sqrt(qn) * (1.0 + 0x1p-50) + a.pmax);
-  const double E2 = ((2.0 * g + 8.0 * u) * N * N + D * 0x1p-125) * (1.0 + 1e-12);
+  const double E2 = ((2.0 * g + 8.0 * u) * N * N + D * 0x1p-110 * (1.0 + N) * (1.0 + N)) * (1.0 + 1e-12);
   const double e1 = (u * (1.0 + 2.0 * u) * N + sqrt(D) * 0x1p-148) * (1.0 + 1e-12);
   const double sh = 1.0 + 2.0 * (D + 3.0) * 0x1p-53;
   double tau = a.cutoff;
@@ -436,7 +530,7 @@ int dev_alloc(T** p, size_t count, hipStream_t s) {
 }  // namespace
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
-  return qp.fp32_screen && qp.metric == M_L2 && qp.eps_factor == 1.0 && qp.k <= (uint64_t)KP && t.rows32 &&
+  return qp.fp32_screen && qp.metric == M_L2 && qp.eps_factor == 1.0 && qp.k <= (uint64_t)KP && t.rows_bf &&
          qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64 &&
          std::getenv("KDT_NO_MFMA_SCREEN") == nullptr;
 }
@@ -472,7 +566,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
     return 100;
   }
   SArgs sa{};
-  sa.rows32 = t.rows32;
+  sa.rows_bf = t.rows_bf;
   sa.pnorm = t.pnorm;
   sa.q = trav.q;
   sa.part_a = part_a;
@@ -483,11 +577,11 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   sa.tiles_per_slice = per;
   sa.n_tiles = n_tiles;
   sa.slices = (uint32_t)slices;
-  const dim3 grid((uint32_t)qtiles, (uint32_t)slices);
+  const dim3 grid((uint32_t)((qtiles + SW - 1) / SW), (uint32_t)slices);
   const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;
-  if (ks == 8) k_screen<8><<<grid, 64, 0, s>>>(sa);
-  else if (ks == 16) k_screen<16><<<grid, 64, 0, s>>>(sa);
-  else k_screen<32><<<grid, 64, 0, s>>>(sa);
+  if (ks == 8) k_screen<8><<<grid, 64 * SW, 0, s>>>(sa);
+  else if (ks == 16) k_screen<16><<<grid, 64 * SW, 0, s>>>(sa);
+  else k_screen<32><<<grid, 64 * SW, 0, s>>>(sa);
 
   SelArgs se{};
   se.part_a = part_a;

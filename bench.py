@@ -54,6 +54,17 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (first entry of /proc/cpuinfo), or "unknown"."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def traffic_from_profiles():
     """HBM bytes per query launch from the committed rocprofv3 PMC summary, if any."""
     f = ROOT / "profiles" / "knn_c2_pmc.md"
@@ -156,6 +167,10 @@ def main():
         t0 = time.perf_counter()
         otree.query(q_h[:m], k=k, threads=threads)
         cpu_qps = m / (time.perf_counter() - t0)
+        m1 = int(min(nq, max(2_000, 2.0 / max(per_q * threads, 1e-9))))  # about 2 s on one core
+        t0 = time.perf_counter()
+        otree.query(q_h[:m1], k=k, threads=1)
+        one_core = m1 / (time.perf_counter() - t0)
         cpu = {
             "value": cpu_qps,
             "unit": "queries/s",
@@ -163,6 +178,8 @@ def main():
             "kind": "port",
             "sample": f"{m} of the {nq} queries, k={k}, on the same tree, {threads} threads "
                       "(oracle = C++ restatement of the reference query, g++ -O3 -ffp-contract=off)",
+            "one_core": {"value": one_core, "sample": f"{m1} queries, 1 thread"},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
         }
 
     line = {

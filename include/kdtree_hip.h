@@ -76,6 +76,10 @@ enum kdt_error {
                                       matrix-core path: skip the traversal of the
                                       queries the certificate does not cover, whose
                                       output rows are then left as they were */
+#define KDT_QUERY_NO_ORDER 0x200u /* keep the given query order: batches of 8192 or more
+                                     queries otherwise run grouped by leaf (a leaf-order
+                                     sort with stream-ordered scratch); results are the
+                                     same either way.  Same bit as KDT_BATCH_NO_ORDER. */
 
 /* Batch flags (kdt_query_batch_device). */
 #define KDT_BATCH_PER_TREE_LAUNCHES 0x100u /* one launch per item instead of one persistent launch */

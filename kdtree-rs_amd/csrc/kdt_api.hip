@@ -70,6 +70,7 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   qp.screen_only = (flags & KDT_QUERY_SCREEN_ONLY) != 0;
   qp.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
   qp.group = kdt::requested_group(flags);
+  qp.allow_order = (flags & KDT_QUERY_NO_ORDER) == 0;
   std::string smsg;
   const int scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
   if (q_on_device && out_on_device && (flags & KDT_QUERY_ASSUME_FINITE)) {

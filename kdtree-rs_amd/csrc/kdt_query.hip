@@ -133,9 +133,7 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
 
   // Large batches run grouped by leaf (see kdt_order.hip).
   uint32_t* order = nullptr;
-  const char* no_order = std::getenv("KDT_NO_QUERY_ORDER");
-  if (qp.allow_order && qp.nq >= ORDER_MIN_QUERIES && qp.nq <= 0xFFFFFFFFull &&
-      !(no_order && *no_order == '1')) {
+  if (qp.allow_order && qp.nq >= ORDER_MIN_QUERIES && qp.nq <= 0xFFFFFFFFull) {
     hipError_t e = hipMallocAsync((void**)&order, qp.nq * 4, s);
     if (e != hipSuccess) {
       msg = std::string("query ordering: ") + hipGetErrorString(e);

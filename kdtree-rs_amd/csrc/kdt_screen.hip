@@ -133,15 +133,62 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, fl
 // (bf16 x bf16 products are exact in FP32; the dropped ql.pl and the split
 // residual are in the error bound).  The accumulator holds D[i][j] for point
 // rows i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
+//
+// A value under the lane's threshold is not inserted at once: it is appended
+// to the lane's buffer of IB slots in LDS, and after every CHK elements, if any
+// lane of the wave holds more than IB - CHK entries, all lanes merge their
+// buffers into their sorted lists together (the insert network then runs once
+// for many lanes, not once for each lane that inserts).  The threshold is the
+// list's 16th value as of the last merge, never below the current one, so a
+// value that skips the buffer is >= the final 16th value: after the last merge
+// each list holds the 16 smallest A of its lane's points, as with immediate
+// inserts.  IB = 16 slots for rows of <= 16 columns and 12 above (16 would
+// leave room for only one workgroup per CU beside the larger point tiles).
+template <int KS>
+constexpr uint32_t insert_slots() { return KS <= 8 ? 16 : 12; }
+constexpr int CHK = 8;
+
 template <int KS>
 struct Screen {
   static constexpr int NS = KS / 8;  // bf16x8 operands per lane half
   static constexpr int RU = KS / 2, RS = RU + 1;
+  static constexpr uint32_t IB = insert_slots<KS>();
   bf16x8 qh[NS], ql[NS];
   float qn;
   float la[KP];     // the KP smallest A seen by this lane, ascending
   uint32_t li[KP];  // their tree positions
-  float thr;        // la[KP - 1]
+  float thr;        // la[KP - 1] at the last merge
+  uint32_t cnt;     // entries in this lane's buffer
+  uint2* buf;       // this lane's buffer: slot e at buf[e * 64 * SW], (A bits, position)
+
+  __device__ __forceinline__ void insert(float A, uint32_t row) {  // dropping the largest
+#pragma unroll
+    for (int e = KP - 1; e > 0; --e) {
+      const bool up = A < la[e - 1];
+      const bool here = !up && A < la[e];
+      la[e] = up ? la[e - 1] : (here ? A : la[e]);
+      li[e] = up ? li[e - 1] : (here ? row : li[e]);
+    }
+    if (A < la[0]) {
+      la[0] = A;
+      li[0] = row;
+    }
+  }
+
+  // Merge every lane's buffer into its list; the whole wave calls it.
+  __device__ __forceinline__ void merge() {
+#pragma unroll 1
+    for (uint32_t e = 0; e < IB; ++e) {
+      if (!__builtin_amdgcn_ballot_w64(e < cnt)) break;
+      if (e < cnt) {
+        const uint2 v = buf[e * 64 * SW];
+        const float A = __uint_as_float(v.x);
+        if (A < la[KP - 1]) insert(A, v.y);
+      }
+    }
+    thr = la[KP - 1];
+    cnt = 0;
+  }
 
   __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const float* pns) {
     uint4 op[2 * NS];
@@ -168,21 +215,19 @@ struct Screen {
       const float4 g4 = pg[r / 4];
       const float pn = (r % 4 == 0) ? g4.x : (r % 4 == 1) ? g4.y : (r % 4 == 2) ? g4.z : g4.w;
       const float A = fmaf(-2.0f, acc[r], qn + pn);  // +inf on padding rows
-      if (A < thr) {  // insert (A, row), dropping the largest
-        const uint32_t row = t * 32 + 8 * (r / 4) + 4 * h + (r % 4);
-#pragma unroll
-        for (int e = KP - 1; e > 0; --e) {
-          const bool up = A < la[e - 1];
-          const bool here = !up && A < la[e];
-          la[e] = up ? la[e - 1] : (here ? A : la[e]);
-          li[e] = up ? li[e - 1] : (here ? row : li[e]);
+      // A wave-uniform branch round the append: a plain divergent `if` here
+      // runs the append under an empty exec mask on every element instead of
+      // skipping it (the barrier keeps the two tests from being merged).
+      if (__builtin_amdgcn_ballot_w64(A < thr)) {
+        __builtin_amdgcn_sched_barrier(0);
+        if (A < thr) {
+          const uint32_t row = t * 32 + 8 * (r / 4) + 4 * h + (r % 4);
+          buf[cnt * 64 * SW] = make_uint2(__float_as_uint(A), row);
+          ++cnt;
         }
-        if (A < la[0]) {
-          la[0] = A;
-          li[0] = row;
-        }
-        thr = la[KP - 1];
       }
+      // cnt <= IB - CHK before each group of CHK elements, so <= IB after it.
+      if (r % CHK == CHK - 1 && __builtin_amdgcn_ballot_w64(cnt > IB - CHK)) merge();
     }
   }
 };
@@ -192,6 +237,7 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
   constexpr int RS = KS / 2 + 1;
   __shared__ uint4 s_rows[2][32 * RS];
   __shared__ __align__(16) float s_pn[2][32];
+  __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
   const uint64_t qtile = (uint64_t)blockIdx.x * SW + (tid >> 6);
@@ -219,6 +265,8 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
     sc.li[e] = 0xFFFFFFFFu;
   }
   sc.thr = INFINITY;
+  sc.cnt = 0;
+  sc.buf = s_buf + tid;
 
   const uint32_t t0 = blockIdx.y * a.tiles_per_slice;
   const uint32_t t1 = min(a.n_tiles, t0 + a.tiles_per_slice);
@@ -254,6 +302,7 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
     }
   }
 #undef KDT_SCREEN_STEP
+  if (active) sc.merge();  // wave-uniform
   if (qvalid) {
     const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
 #pragma unroll
@@ -531,8 +580,7 @@ int dev_alloc(T** p, size_t count, hipStream_t s) {
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.metric == M_L2 && qp.eps_factor == 1.0 && qp.k <= (uint64_t)KP && t.rows_bf &&
-         qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64 &&
-         std::getenv("KDT_NO_MFMA_SCREEN") == nullptr;
+         qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64;
 }
 
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg) {

@@ -27,6 +27,7 @@ KDT_QUERY_FP32_SCREEN = 0x1
 KDT_QUERY_ASSUME_FINITE = 0x2
 KDT_QUERY_MATRIX_SCREEN = 0x8
 KDT_QUERY_GROUP_SHIFT = 4
+KDT_QUERY_NO_ORDER = 0x200
 KDT_BATCH_PER_TREE_LAUNCHES = 0x100
 KDT_BATCH_NO_ORDER = 0x200
 

@@ -191,13 +191,24 @@ def test_eps_box_order_and_cutoff_together():
     _query_case(np.concatenate([g, g[::3]]), g[::7] + 0.5, k=10, leafsize=4)
 
 
-@pytest.mark.parametrize("order", ["1", "0"])
-def test_query_ordering_is_invisible(monkeypatch, order):
-    # Batches >= 8192 queries run in leaf order; results must not change.
-    monkeypatch.setenv("KDT_NO_QUERY_ORDER", order)
+@pytest.mark.parametrize("no_order", [True, False])
+def test_query_ordering_is_invisible(no_order):
+    # Batches >= 8192 queries run in leaf order unless KDT_QUERY_NO_ORDER
+    # (0x200) is set; results must not change.
+    import ctypes
+
     data = np.random.default_rng(120).uniform(size=(50_000, 3))
     queries = np.random.default_rng(121).uniform(size=(20_000, 3))
-    _query_case(data, queries, k=16)
+    od, oi = OracleTree(data, 32).query(queries, k=16)
+    tree = KDTree(data, leafsize=32)
+    out_d = np.empty((queries.shape[0], 16))
+    out_i = np.empty((queries.shape[0], 16), dtype=np.int64)
+    rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(queries.ctypes.data), queries.shape[0], 3, 16, 2.0,
+                             np.inf, 0.0, 0x200 if no_order else 0, ctypes.c_void_p(out_d.ctypes.data),
+                             ctypes.c_void_p(out_i.ctypes.data))
+    assert rc == 0, tree._lib.kdt_last_error().decode()
+    np.testing.assert_array_equal(out_i, oi)
+    assert out_d.tobytes() == od.tobytes()
 
 
 @pytest.mark.parametrize("group", [8, 16, 32, 64])

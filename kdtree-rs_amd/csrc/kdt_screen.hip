@@ -11,10 +11,11 @@
 //
 // Steps for a query matrix against one tree:
 //   1. k_screen: for a tile of 32 queries and a slice of the points, the
-//      matrix cores form q.p from bf16 halves of the FP32-rounded rows (three
-//      v_mfma_f32_32x32x16_bf16 per 16 coordinates) and each lane keeps the KP = 16
-//      smallest  A = |q|^2 + |p|^2 - 2 q.p  of the points it sees, with the
-//      points' tree positions;
+//      matrix cores form  A = |q|^2 + |p|^2 - 2 q.p  from bf16 halves of the
+//      FP32-rounded rows and norms (three v_mfma_f32_32x32x16_bf16 per 16
+//      coordinates for -2 q.p, one more that adds the two norms) and each lane
+//      keeps the KP = 16 smallest A of the points it sees, with the points'
+//      tree positions;
 //   2. k_screen_select: the k-th smallest A over all of a query's lists gives
 //      A_k, and from it a threshold tau_A such that every point whose FP64
 //      distance could be among the k best has A <= tau_A (error bound
@@ -37,11 +38,14 @@
 // |p~| of the tree, N = |q~| + P).  Splitting an FP32 x into bf16 halves
 // (precision 8 bits) leaves |x - xh - xl| <= 2^-16 |x| and |xl| <= 2^-8 |x|
 // (1 + 2^-8), so the three products miss at most 3.1 2^-16 |x| |y| per
-// coordinate; the matrix core adds the 3 D exact products in FP32:
-//   |A - |q~ - p~|^2| <= E2 = (2 g + 8u) N^2 + D 2^-110 (1 + N)^2,
-//     g = 3.1 2^-16 + 1.03 (3D+4)u / (1 - (3D+4)u)
-//     (the dot product, FP32 rounding of |q~|^2, |p~|^2, the sum and the final
-//      fma; flushed subnormal halves and products);
+// coordinate (x = -2 q~_i, exact) and the split norms qn = fl32(|q~|^2),
+// pn = fl32(|p~|^2) at most 2^-16 (qn + pn); the matrix core adds the 3 D + 4
+// exact terms in FP32, whose magnitudes sum to at most (1 + 2^-6) N^2:
+//   |A - |q~ - p~|^2| <= (1.02 gamma + 3.1 2^-16 + 1.01 u) N^2 + flushes
+//     <= E2 = (2 g + 8u) N^2 + D 2^-110 (1 + N)^2,
+//     gamma = (3D+3)u / (1 - (3D+3)u),  g = 3.1 2^-16 + 1.03 (3D+4)u / (1 - (3D+4)u)
+//     (the split residuals, FP32 rounding of the norms, the accumulation;
+//      flushed subnormal halves and products);
 //   | |q~ - p~| - |q - p| | <= e1 = u (1 + 2u) N + sqrt(D) 2^-148;
 //   the FP64 reduced distance d of the reference satisfies d / s <= |q-p|^2
 //     <= d s with s = 1 + 2 (D + 3) 2^-53.
@@ -153,8 +157,8 @@ struct Screen {
   static constexpr int NS = KS / 8;  // bf16x8 operands per lane half
   static constexpr int RU = KS / 2, RS = RU + 1;
   static constexpr uint32_t IB = insert_slots<KS>();
-  bf16x8 qh[NS], ql[NS];
-  float qn;
+  bf16x8 qh[NS], ql[NS];  // -2 q~, split
+  bf16x8 qnb;             // norm block, B side: (qn_hi, qn_lo, 1, 1, 0...) in half 0, zero in half 1
   float la[KP];     // the KP smallest A seen by this lane, ascending
   uint32_t li[KP];  // their tree positions
   float thr;        // la[KP - 1] at the last merge
@@ -191,37 +195,51 @@ struct Screen {
   }
 
   __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const float* pns) {
-    uint4 op[2 * NS];
     const uint4* rb = rows + j * RS + h * NS;
-#pragma unroll
-    for (int v = 0; v < NS; ++v) {
-      op[v] = rb[v];
-      op[NS + v] = rb[2 * NS + v];
-    }
-    float4 pg[4];
-#pragma unroll
-    for (int g = 0; g < 4; ++g) pg[g] = reinterpret_cast<const float4*>(pns)[2 * g + h];
+    // Norm block, A side: (1, 1, pn_hi, pn_lo, 0...) of row j in half 0, so one
+    // more instruction adds qn + pn (four exact terms) to every element and the
+    // accumulator ends as A itself.  Padding rows: pn = +inf, pn_lo = 0, A = +inf.
+    const float pnj = pns[j];
+    const __bf16 pnh = (__bf16)pnj;
+    const __bf16 pnl = (__bf16)(__builtin_isinf(pnj) ? 0.0f : pnj - (float)pnh);
+    const uint32_t pw = (uint32_t)__builtin_bit_cast(uint16_t, pnh) |
+                        ((uint32_t)__builtin_bit_cast(uint16_t, pnl) << 16);
+    const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u);
+    // The point operands of 16 coordinates are read one group ahead of their
+    // three products (left to itself the scheduler reads all 2 NS operands
+    // first, and the extra registers push the query halves to scratch).
     floatx16 acc = {};
+    uint4 ch = rb[0], cl = rb[2 * NS];
 #pragma unroll
     for (int v = 0; v < NS; ++v) {
-      const bf16x8 ph = __builtin_bit_cast(bf16x8, op[v]);
-      const bf16x8 pl = __builtin_bit_cast(bf16x8, op[NS + v]);
+      uint4 nh = ch, nl = cl;
+      if (v + 1 < NS) {
+        nh = rb[v + 1];
+        nl = rb[2 * NS + v + 1];
+      }
+      const bf16x8 ph = __builtin_bit_cast(bf16x8, ch);
+      const bf16x8 pl = __builtin_bit_cast(bf16x8, cl);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, qh[v], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, ql[v], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pl, qh[v], acc, 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      ch = nh;
+      cl = nl;
     }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb, acc, 0, 0, 0);
+    // Element r is row rowbase | (8 (r / 4) + r % 4): disjoint bits, one
+    // immediate OR, nothing per element to keep in registers across tiles.
+    const uint32_t rowbase = t * 32 + 4 * h;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const float4 g4 = pg[r / 4];
-      const float pn = (r % 4 == 0) ? g4.x : (r % 4 == 1) ? g4.y : (r % 4 == 2) ? g4.z : g4.w;
-      const float A = fmaf(-2.0f, acc[r], qn + pn);  // +inf on padding rows
+      const float A = acc[r];
       // A wave-uniform branch round the append: a plain divergent `if` here
       // runs the append under an empty exec mask on every element instead of
       // skipping it (the barrier keeps the two tests from being merged).
       if (__builtin_amdgcn_ballot_w64(A < thr)) {
         __builtin_amdgcn_sched_barrier(0);
         if (A < thr) {
-          const uint32_t row = t * 32 + 8 * (r / 4) + 4 * h + (r % 4);
+          const uint32_t row = rowbase | (8 * (r / 4) + (r % 4));
           buf[cnt * 64 * SW] = make_uint2(__float_as_uint(A), row);
           ++cnt;
         }
@@ -253,12 +271,20 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
     const uint32_t c = h * KS + s;
     const double raw = a.q[min(qrow, a.nq - 1) * a.D + min(c, a.D - 1)];
     const float v = (qvalid && c < a.D) ? (float)raw : 0.0f;
-    const __bf16 hi = (__bf16)v;
+    const float w = -2.0f * v;  // exact
+    const __bf16 hi = (__bf16)w;
     sc.qh[s / 8][s % 8] = hi;
-    sc.ql[s / 8][s % 8] = (__bf16)(v - (float)hi);
+    sc.ql[s / 8][s % 8] = (__bf16)(w - (float)hi);
     qn_part += (double)v * (double)v;
   }
-  sc.qn = (float)(qn_part + __shfl_xor(qn_part, 32));
+  {
+    const float qn = (float)(qn_part + __shfl_xor(qn_part, 32));
+    const __bf16 qnh = (__bf16)qn;
+    const __bf16 qnl = (__bf16)(qn - (float)qnh);
+    const uint32_t qw = (uint32_t)__builtin_bit_cast(uint16_t, qnh) |
+                        ((uint32_t)__builtin_bit_cast(uint16_t, qnl) << 16);
+    sc.qnb = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(qw, 0x3F803F80u, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u));
+  }
 #pragma unroll
   for (int e = 0; e < KP; ++e) {
     sc.la[e] = INFINITY;

@@ -1,10 +1,11 @@
 """The matrix-core screen's error bound, checked against exact arithmetic.
 
-The screen (kdt_screen.hip) rounds the rows to FP32 (q~, p~), splits each
-value x into bf16 halves xh = bf16(x), xl = bf16(x - xh), forms
-q~.p~ ~ qh.ph + ql.ph + qh.pl (bf16 products, exact in FP32, summed in FP32),
-A = fl32(|q~|^2 + |p~|^2 - 2 dot), and relies on two consequences of its bound,
-with N = |q~| + |p~|, u = 2^-24, s = 1 + 2(D+3)2^-53,
+The screen (kdt_screen.hip) rounds the rows to FP32 (q~, p~) and their squared
+norms to FP32 (qn, pn), splits each value x into bf16 halves xh = bf16(x),
+xl = bf16(x - xh), and has the matrix cores sum in FP32 the 3D exact products
+of -2 q~ and p~ (xh.yh + xl.yh + xh.yl per coordinate) and the four exact terms
+qn_h + qn_l + pn_h + pn_l: that sum is A.  It relies on two consequences of
+its bound, with N = |q~| + |p~|, u = 2^-24, s = 1 + 2(D+3)2^-53,
 g = 3.1 2^-16 + 1.03 (3D+4)u / (1 - (3D+4)u):
 
   (upper)     d <= U(A) = s (sqrt(max(A, 0) + E2) + e1)^2
@@ -15,7 +16,7 @@ where d is the reference's FP64 reduced distance (the oracle's point_dist, the
 Streamed / Folded summation of src/kernel.rs).  (upper) makes tau an upper bound
 of the k-th distance; (inclusion) puts every top-k point among the candidates.
 Here A is formed as the kernel forms it (bf16 rounding emulated bit for bit,
-round to nearest even), with the 3D products accumulated in FP32 in several
+round to nearest even), with the 3D + 4 terms accumulated in FP32 in several
 orders (the matrix cores' internal order is not specified), on inputs chosen to
 stress the bound: near-equal rows with large norms (cancellation in
 |q|^2 + |p|^2 - 2 q.p), rows whose split residuals all add up, mixed
@@ -60,9 +61,8 @@ def split_terms(a, b):
     return out
 
 
-def dot32(a, b, order):
-    """The screen's contraction: split products summed in FP32 in a given order."""
-    v = split_terms(a, b)
+def fsum32(v, order):
+    """FP32 sum of float32 terms in a given order."""
     if order == "seq":
         acc = f32(0.0)
         for t in v:
@@ -89,8 +89,14 @@ def dot32(a, b, order):
 def screen_A(q32, p32, order):
     qn = f32(float(np.sum(q32.astype(np.float64) ** 2)))
     pn = f32(float(np.sum(p32.astype(np.float64) ** 2)))
-    dot = dot32(q32, p32, order)
-    return float(f32(-2.0 * float(dot) + float(f32(qn + pn))))  # fmaf: one rounding of the exact value
+    norms = []
+    for x in (qn, pn):  # the norm block: x_h * 1 + x_l * 1
+        xh = bf16(x)
+        norms += [xh, bf16(f32(x - xh))]
+    prods = split_terms((np.float32(-2.0) * q32).astype(np.float32), p32)
+    if order == "norms-first":
+        return float(fsum32(norms + prods, "seq"))
+    return float(fsum32(prods + norms, order))  # the kernel issues the norm block last
 
 
 def bounds(D, nq, npn, cancellation=True, split=True):
@@ -128,7 +134,7 @@ def cases():
     return out
 
 
-@pytest.mark.parametrize("order", ["seq", "fma", "pairs", "tree"])
+@pytest.mark.parametrize("order", ["seq", "fma", "pairs", "tree", "norms-first"])
 def test_screen_bound_holds(order):
     for D, q, p in cases():
         q32, p32 = q.astype(np.float32), p.astype(np.float32)

@@ -282,9 +282,11 @@ def run_c3(args):
 def run_c4(args):
     """256 trees (n log-uniform in [1e3,1e6], d in {2,3,8,16}, float32 uniform
     promoted to float64), per-tree query counts log-uniform in [1, 1e5], k in
-    {1, 8, 32}; leafsize autotuned over {8,16,32,64} and lanes per query over
-    the widths that hold k.  One persistent launch
-    versus one launch per tree versus the per-tree launches as a HIP graph."""
+    {1, 8, 32}; leafsize autotuned over {8,16,32,64}, lanes per query over the
+    widths that hold k, and the matrix-core screen where it applies.  Shared
+    persistent launches (items of 8192+ queries on their own) versus one launch
+    sequence per tree versus the per-tree launches captured as a HIP graph (a
+    pure enqueue: no leaf ordering and no matrix-core screen)."""
     import torch
 
     from kdtree import forest as F
@@ -303,9 +305,12 @@ def run_c4(args):
     tune_s = time.perf_counter() - t0
     batch = F.Batch(items)
     nq = batch.n_queries
-    res = {}
+    res, outs = {}, {}
     for mode in ("per_tree", "graph", "persistent"):
         res[mode] = nq / _time_steps(lambda: batch.run(mode), args.steps, args.warmup)
+        outs[mode] = [(od.clone(), oi.clone()) for _, _, _, od, oi in batch.items]
+    modes_equal = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+                      for m in ("per_tree", "graph") for a, b in zip(outs[m], outs["persistent"]))
     batch.run("persistent")
     torch.cuda.synchronize()
     parity = True
@@ -325,6 +330,8 @@ def run_c4(args):
         "persistent_vs_per_tree": res["persistent"] / res["per_tree"],
         "autotuned_leafsize_counts": leafs, "autotuned_lanes_per_query_counts": groups, "autotune_s": tune_s,
         "parity_first_8_trees_bit_exact": parity,
+        "all_items_bitwise_equal_across_modes": bool(modes_equal),
+        "matrix_screen_trees": sum(1 for tree, _, _ in items if tree.flags),
     }
     for tree, _, _ in items:
         tree.free()

@@ -83,8 +83,10 @@ enum kdt_error {
 
 /* Batch flags (kdt_query_batch_device). */
 #define KDT_BATCH_PER_TREE_LAUNCHES 0x100u /* one launch per item instead of one persistent launch */
-#define KDT_BATCH_NO_ORDER 0x200u          /* keep the given query order (no leaf-order sort), so
-                                              the call enqueues kernels only */
+#define KDT_BATCH_NO_ORDER 0x200u          /* keep the given query order (no leaf-order sort) and
+                                              skip the matrix-core screen (items it would take run
+                                              the traversal, same results): the call enqueues
+                                              kernels only, so it can be captured in a graph */
 
 /* One item of a multi-tree batch: queries (device, n_queries x ndim, ndim =
  * the tree's) against one tree, k neighbours, outputs n_queries x k (device). */
@@ -98,6 +100,9 @@ typedef struct kdt_batch_item {
   int64_t* out_idx;
   uint32_t group;  /* lanes per query for this item's tree (8, 16, 32, 64; 0 = from the
                       flags or from k), e.g. the width an autotuner picked */
+  uint32_t flags;  /* KDT_QUERY_FP32_SCREEN | KDT_QUERY_MATRIX_SCREEN for this item on top
+                      of the call's flags (other bits ignored), e.g. an autotuner's
+                      per-tree choice of the matrix-core screen; 0 = the call's */
 } kdt_batch_item;
 
 /* Node record exported for inspection (24 bytes, the reference's Node budget,

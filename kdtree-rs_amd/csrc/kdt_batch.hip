@@ -148,6 +148,18 @@ int batch_fail(int code, std::string m) {
 }
 }  // namespace
 
+namespace kdt {
+namespace {
+constexpr uint64_t SINGLE_MIN = 8192;
+
+// An item's own screen flags, on top of the call's.
+void item_flags(const kdt_batch_item& it, QueryParams& qp) {
+  qp.fp32_screen = qp.fp32_screen || (it.flags & KDT_QUERY_FP32_SCREEN) != 0;
+  qp.matrix_screen = qp.matrix_screen || (it.flags & KDT_QUERY_MATRIX_SCREEN) != 0;
+}
+}  // namespace
+}  // namespace kdt
+
 extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_items, double p,
                                       double max_distance, double eps, uint32_t flags, void* stream) {
   using namespace kdt;
@@ -162,6 +174,11 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   base.group = requested_group(flags);
   const bool per_tree = (flags & KDT_BATCH_PER_TREE_LAUNCHES) != 0;
   const bool no_order = (flags & KDT_BATCH_NO_ORDER) != 0;
+  // KDT_BATCH_NO_ORDER makes the call a pure enqueue (capturable in a graph):
+  // no leaf-order sort and no matrix-core screen, both of which take
+  // stream-ordered scratch whose lifetime ends inside the call.  Items that
+  // would take the screen run the traversal, with the same results.
+  base.allow_scratch = !no_order;
 
   // Validate; split items into width classes that run in shared launches and
   // the rest (general p, other widths, k > 64, the matrix-core screen), which
@@ -185,7 +202,14 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     QueryParams probe = base;
     probe.k = it.k;
     probe.nq = it.n_queries;
-    if (per_tree || base.metric == M_LP || c < 0 || it.k > 64 || screen_applies(t, probe)) single.push_back(i);
+    item_flags(it, probe);
+    // Items of SINGLE_MIN queries or more take their own launch sequence, as
+    // alone (own lanes per query, own leaf ordering, uniform kernel arguments):
+    // their launch overhead is amortised, and in the shared launch they ran
+    // slower.  The shared persistent launches keep the many small items.
+    if (per_tree || base.metric == M_LP || c < 0 || it.k > 64 || it.n_queries >= SINGLE_MIN ||
+        screen_applies(t, probe))
+      single.push_back(i);
     else cls_items[c].push_back(i);
   }
 
@@ -194,6 +218,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     QueryParams qp = base;
     qp.k = it.k;
     qp.nq = it.n_queries;
+    item_flags(it, qp);
     qp.allow_order = !no_order;
     if (it.group) qp.group = (int)it.group;
     rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, s, msg);
@@ -202,11 +227,12 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
 
   // Shared launches: items numbered class by class, so each class is one
   // contiguous range of the batch's query numbers (and of the sorted order).
+  constexpr int NC = 6;
   std::vector<BItem> h;
-  uint64_t first[6] = {0}, count[6] = {0}, kmax[6] = {0}, gmax[6] = {0};
-  uint32_t stk[6] = {2, 2, 2, 2, 2, 2};
+  uint64_t first[NC] = {0}, count[NC] = {0}, kmax[NC] = {0}, gmax[NC] = {0};
+  uint32_t stk[NC] = {2, 2, 2, 2, 2, 2};
   uint64_t total = 0;
-  for (int c = 0; c < 6; ++c) {
+  for (int c = 0; c < NC; ++c) {
     first[c] = total;
     for (uint64_t i : cls_items[c]) {
       const kdt_batch_item& it = items[i];
@@ -231,7 +257,7 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   unsigned long long* counters = nullptr;
   uint32_t* order = nullptr;
   hipError_t e = hipMallocAsync((void**)&d_items, h.size() * sizeof(BItem), s);
-  if (e == hipSuccess) e = hipMallocAsync((void**)&counters, 6 * sizeof(unsigned long long), s);
+  if (e == hipSuccess) e = hipMallocAsync((void**)&counters, NC * sizeof(unsigned long long), s);
   if (e == hipSuccess) e = hipMemcpyAsync(d_items, h.data(), h.size() * sizeof(BItem), hipMemcpyHostToDevice, s);
   if (e == hipSuccess && !no_order && total >= 8192) {
     // One sort over the batch: key = (item, leaf reached by the near path).
@@ -262,24 +288,26 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     if (tmp) (void)hipFreeAsync(tmp, s);
   }
   if (e == hipSuccess) {
-    k_zero<<<1, 64, 0, s>>>(counters, 6);
+    static_assert(NC <= 64, "one thread per class counter");
+    k_zero<<<1, 64, 0, s>>>(counters, NC);
     const bool screen = base.fp32_screen && base.metric == M_L2;
     const uint32_t n = (uint32_t)h.size();
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < NC; ++c) {
       if (!count[c]) continue;
+      const uint64_t w = widths[c];
       // Lanes per query: the widest requested by the class's items (an
       // autotuner's per-tree choice) when it holds every item's k, else from k.
       int G = kmax[c] <= 16 ? 16 : kmax[c] <= 32 ? 32 : 64;
       if ((gmax[c] == 8 || gmax[c] == 16 || gmax[c] == 32 || gmax[c] == 64) && gmax[c] >= kmax[c]) G = (int)gmax[c];
       switch (base.metric) {
         case M_L1:
-          launch_class_width<M_L1>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
+          launch_class_width<M_L1>(w, G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
           break;
         case M_L2:
-          launch_class_width<M_L2>(widths[c], G, screen, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
+          launch_class_width<M_L2>(w, G, screen, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
           break;
         default:
-          launch_class_width<M_LINF>(widths[c], G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
+          launch_class_width<M_LINF>(w, G, false, d_items, n, order, first[c], count[c], counters + c, stk[c], s);
           break;
       }
     }

@@ -68,10 +68,24 @@ struct QueryParams {
   bool screen_only = false;    // diagnostic: no traversal after the matrix-core screen
   bool matrix_screen = false;  // matrix-core screen also for 16 <= ndim < 32
   int group = 0;               // lanes per query requested (8, 16, 32, 64), 0 = from k
-  bool allow_order = true;  // large batches may run in leaf order
+  bool allow_order = true;    // large batches may run in leaf order
+  bool allow_scratch = true;  // stream-ordered scratch allowed (the matrix-core screen needs it);
+                              // false for a pure enqueue, e.g. a launch sequence under graph capture
 };
 
 // Lanes per query requested by KDT_QUERY_GROUP bits of the query flags (0 = none).
+// Lanes per query of the traversal: the narrowest group that holds the k list
+// (16 for k <= 16, 32 for k <= 32, 64 above) -- narrow groups keep more queries
+// in flight per wavefront, which measured faster than covering a whole leaf per
+// pass -- unless a requested width (query flags or a batch item, picked per
+// tree by timing) still holds k.
+inline int group_width(uint64_t k, int requested) {
+  int g = k <= 16 ? 16 : k <= 32 ? 32 : 64;
+  if ((requested == 8 || requested == 16 || requested == 32 || requested == 64) && (uint64_t)requested >= k)
+    g = requested;
+  return g;
+}
+
 inline int requested_group(uint32_t flags) {
   const uint32_t g = (flags >> 4) & 7u;
   return g >= 1 && g <= 4 ? (4 << g) : 0;

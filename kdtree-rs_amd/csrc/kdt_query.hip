@@ -23,17 +23,6 @@ constexpr uint64_t ORDER_MIN_QUERIES = 8192; // smaller batches run in the given
 
 bool compiled_width(uint64_t d) { return d == 1 || d == 2 || d == 3 || d == 4 || d == 8 || d == 16; }
 
-// Lanes per query: the narrowest group that holds the k list.  Narrow groups
-// keep more queries in flight per wavefront, which measured faster than
-// covering a whole leaf per pass (a 32-row leaf takes two passes at G = 16).
-// A requested width (query flags, picked per tree by timing) wins when it
-// still holds k.
-int group_width(uint64_t k, int requested) {
-  int g = k <= 16 ? 16 : k <= 32 ? 32 : 64;
-  if ((requested == 8 || requested == 16 || requested == 32 || requested == 64) && (uint64_t)requested >= k)
-    g = requested;
-  return g;
-}
 }  // namespace
 
 std::string rust_f64(double x) {

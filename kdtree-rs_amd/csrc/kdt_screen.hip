@@ -605,7 +605,8 @@ int dev_alloc(T** p, size_t count, hipStream_t s) {
 }  // namespace
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
-  return qp.fp32_screen && qp.metric == M_L2 && qp.eps_factor == 1.0 && qp.k <= (uint64_t)KP && t.rows_bf &&
+  return qp.fp32_screen && qp.allow_scratch && qp.metric == M_L2 && qp.eps_factor == 1.0 &&
+         qp.k <= (uint64_t)KP && t.rows_bf &&
          qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64;
 }
 

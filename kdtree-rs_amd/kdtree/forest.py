@@ -42,6 +42,7 @@ class BatchItem(ctypes.Structure):
         ("out_dist", ctypes.c_void_p),
         ("out_idx", ctypes.c_void_p),
         ("group", ctypes.c_uint32),  # lanes per query, 0 = automatic
+        ("flags", ctypes.c_uint32),  # the item's own screen flags, 0 = the call's
     ]
 
 
@@ -79,6 +80,7 @@ class DeviceTree:
     d: int
     leafsize: int
     group: int = 0  # lanes per query chosen by the autotuner (0 = automatic)
+    flags: int = 0  # screen flags chosen by the autotuner (0 = none)
     tune_ms: dict = field(default_factory=dict)
 
     def free(self):
@@ -111,6 +113,7 @@ def query_device(tree: DeviceTree, q, k: int, out_d=None, out_i=None, flags: int
     lib = _lib()
     if tree.group and not flags & (7 << KDT_QUERY_GROUP_SHIFT):
         flags |= group_flags(tree.group)
+    flags |= tree.flags
     nq = q.shape[0]
     if out_d is None:
         out_d = torch.empty((nq, k), dtype=torch.float64, device=q.device)
@@ -124,12 +127,14 @@ def query_device(tree: DeviceTree, q, k: int, out_d=None, out_i=None, flags: int
 def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), groups=None,
                    reps: int = 3) -> DeviceTree:
     """Build with each leafsize, time `reps` queries of the sample with each
-    group width (lanes per query that hold k), keep the fastest pair.
+    group width (lanes per query that hold k), keep the fastest pair; then,
+    where the matrix-core screen applies (16 <= d <= 64, k <= 16, for queries
+    with p = 2 and eps = 0), time it too and keep it in `flags` if faster.
 
     The leafsize changes the tree, and for 4- and 8-dimensional rows the
     reference's summation order inside a leaf, so the chosen leafsize is part
     of the result's definition: the oracle must use the same one.  The group
-    width changes only the speed.
+    width and the screen change only the speed.
     """
     torch = _torch()
     q = torch.from_numpy(np.ascontiguousarray(sample_queries, np.float64)).cuda()
@@ -158,6 +163,18 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
         if not keep:
             tree.free()
     best.group = best_g
+    if 16 <= best.d <= 64 and k <= 16:
+        sflags = KDT_QUERY_FP32_SCREEN | KDT_QUERY_MATRIX_SCREEN
+        out_d, out_i = query_device(best, q, k, flags=KDT_QUERY_ASSUME_FINITE | sflags)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            query_device(best, q, k, out_d, out_i, flags=KDT_QUERY_ASSUME_FINITE | sflags)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        timings["matrix_screen"] = ms
+        if ms < best_ms:
+            best.flags = sflags
     best.tune_ms = timings
     return best
 
@@ -174,7 +191,7 @@ class Batch:
             out_i = torch.empty((q.shape[0], k), dtype=torch.int64, device=q.device)
             self.items.append((tree, q, k, out_d, out_i))
             self.arr[j] = BatchItem(tree.handle.value, q.data_ptr(), q.shape[0], tree.d, k, out_d.data_ptr(),
-                                    out_i.data_ptr(), tree.group)
+                                    out_i.data_ptr(), tree.group, tree.flags)
         self.graph = None
 
     @property

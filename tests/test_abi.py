@@ -92,3 +92,28 @@ def test_too_many_nodes_message():
     assert msg == (
         "tree would need 4294967297 nodes, which exceeds the 32-bit node index; increase leafsize"
     )
+
+
+def test_batch_item_layout_matches_header(tmp_path):
+    # The Python mirror of kdt_batch_item (kdtree.forest.BatchItem) against the
+    # C compiler's layout of the header's struct: size and every field offset.
+    import shutil
+    import subprocess
+
+    from kdtree.forest import BatchItem
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = [f for f, _ in BatchItem._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stddef.h>\n#include <stdio.h>\n#include "kdtree_hip.h"\nint main(void) {\n'
+        '  printf("%zu\\n", sizeof(kdt_batch_item));\n'
+        + "".join(f'  printf("%zu\\n", offsetof(kdt_batch_item, {f}));\n' for f in fields)
+        + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(BatchItem)
+    assert out[1:] == [getattr(BatchItem, f).offset for f in fields]

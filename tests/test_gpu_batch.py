@@ -72,3 +72,36 @@ def test_batch_fp32_screen_matches(forest):
     batch.run("persistent", extra_flags=0x1)
     torch.cuda.synchronize()
     check_all(batch, host)
+
+
+def test_item_screen_flags_match_oracle(forest):
+    # Items whose tree asks for the matrix-core screen (the per-item flags an
+    # autotuner sets) run on their own launch sequence; every mode, the graph
+    # replays included, still gives the oracle's answer for every item.
+    import torch
+
+    from kdtree import forest as F
+
+    batch, host = forest
+    sflags = F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_MATRIX_SCREEN
+    trees = [t for t, _, _, _, _ in batch.items]
+    saved = [t.flags for t in trees]
+    forced = 0
+    for tree, _, k, _, _ in batch.items:
+        if 16 <= tree.d <= 64 and k <= 16:
+            tree.flags = sflags
+            forced += 1
+    assert forced > 0, "no item the matrix-core screen applies to"
+    try:
+        b2 = F.Batch([(t, q, k) for t, q, k, _, _ in batch.items])
+        for mode in ("persistent", "per_tree", "graph", "graph", "graph"):
+            for _, _, _, od, oi in b2.items:
+                od.fill_(-1.0)
+                oi.fill_(-1)
+            torch.cuda.synchronize()
+            b2.run(mode)
+            torch.cuda.synchronize()
+            check_all(b2, host)
+    finally:
+        for t, f in zip(trees, saved):
+            t.flags = f

@@ -105,3 +105,44 @@ def test_item_screen_flags_match_oracle(forest):
     finally:
         for t, f in zip(trees, saved):
             t.flags = f
+
+
+@pytest.mark.parametrize("p,eps,max_distance", [(1.0, 0.0, np.inf), (np.inf, 0.0, np.inf), (2.0, 0.5, np.inf),
+                                                (2.0, 0.0, 0.3), (1.0, 0.25, 0.5)])
+@pytest.mark.parametrize("per_tree", [False, True])
+def test_batch_metrics_eps_cutoff_match_oracle(p, eps, max_distance, per_tree):
+    # The batch call's p, eps and max_distance apply to every item, through
+    # the shared launches and through per-tree launches alike.
+    import ctypes
+
+    import torch
+
+    from kdtree import forest as F
+
+    rng = np.random.default_rng(300)
+    specs = [(3000, 2, 700, 8), (5000, 3, 900, 1), (2000, 8, 400, 32), (1500, 16, 300, 8), (40, 3, 50, 16),
+             (9000, 3, 9000, 4)]
+    items, host = [], []
+    for n, d, q, k in specs:
+        data = rng.uniform(size=(n, d))
+        queries = rng.uniform(size=(q, d))
+        tree = F.build(torch.from_numpy(data).cuda(), 16)
+        items.append((tree, torch.from_numpy(queries).cuda(), k))
+        host.append((data, queries, k))
+    batch = F.Batch(items)
+    lib = F._lib()
+    flags = F.KDT_BATCH_PER_TREE_LAUNCHES if per_tree else 0
+    rc = lib.kdt_query_batch_device(batch.arr, len(batch.items), ctypes.c_double(p), ctypes.c_double(max_distance),
+                                    ctypes.c_double(eps), ctypes.c_uint32(flags), None)
+    assert rc == 0, lib.kdt_last_error().decode()
+    torch.cuda.synchronize()
+    try:
+        for j, (data, queries, k) in enumerate(host):
+            od, oi = OracleTree(data, 16).query(queries, k=k, p=p, eps=eps,
+                                                max_distance=None if np.isinf(max_distance) else max_distance)
+            gd, gi = batch.results(j)
+            np.testing.assert_array_equal(gi.cpu().numpy(), oi, err_msg=f"item {j}")
+            assert gd.cpu().numpy().tobytes() == od.tobytes(), f"item {j} distances"
+    finally:
+        for tree, _, _ in items:
+            tree.free()

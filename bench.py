@@ -425,7 +425,9 @@ def run_c5(args):
 
     plain   each request alone, one after another, through kdt_query (host
             buffers in and out, the FP64 traversal kernel): what a caller of
-            the reference API gets from the plain GPU kernel;
+            the reference API gets from the plain GPU kernel; and the same
+            requests with their queries resident in HBM, one kdt_query_device
+            call per tree (the plain kernel alone);
     served  the continuous-batching loop (kdtree.serve.Server): requests are
             admitted while others run, coalesced into multi-tree launches on
             several streams, FP32 screens (the matrix-core screen for d = 64,
@@ -499,6 +501,27 @@ def run_c5(args):
     plain_lat = np.array([dt for _, dt, _, _ in plain]) * 1e3
     sample = [r for r, _, _, _ in plain]
 
+    # Plain kernel, device-resident: the same sample, each tree's requests
+    # concatenated into one kdt_query_device call (the FP64 traversal, no
+    # screen, queries already in HBM): the plain GPU kernel without the
+    # per-request host copies and calls.
+    by_tree = {}
+    for j, r in enumerate(sample):
+        by_tree.setdefault(r.tree, []).append(j)
+    dev = {t: torch.from_numpy(np.concatenate([sample[j].queries for j in js])).cuda() for t, js in by_tree.items()}
+    for t in dev:
+        F.query_device(trees[t], dev[t][:1], k)
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    pk_out = {t: F.query_device(trees[t], dev[t], k) for t in dev}
+    torch.cuda.synchronize()
+    pk_t = time.perf_counter() - s0
+    pk_exact = all(
+        np.array_equal(pk_out[t][1].cpu().numpy(), np.concatenate([plain[j][3] for j in js]))
+        and pk_out[t][0].cpu().numpy().tobytes() == np.concatenate([plain[j][2] for j in js]).tobytes()
+        for t, js in by_tree.items())
+    del dev, pk_out
+
     def serve(reqs):
         server = Server(trees, n_streams=4, hbm_budget_bytes=C5["hbm_budget_bytes"],
                         max_queries_per_launch=1 << 18, flags=serve_flags)
@@ -540,8 +563,12 @@ def run_c5(args):
         "plain": {"queries_per_s": plain_q / plain_t, "requests": len(plain), "queries": plain_q,
                   "note": "first requests in arrival order, each cut to <= 2048 queries",
                   "p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99))},
+        "plain_kernel_device_resident": {"queries_per_s": plain_q / pk_t, "seconds": pk_t,
+                                         "note": "same sample, one kdt_query_device call per tree, FP64 traversal",
+                                         "exact_vs_plain_bitwise": bool(pk_exact)},
         "served_same_requests": served_sample,
         "served_vs_plain_same_requests": served_sample["queries_per_s"] / (plain_q / plain_t),
+        "served_vs_plain_kernel_same_requests": served_sample["queries_per_s"] / (plain_q / pk_t),
         "served_saturation": sat, "served_at_load": loads,
         "d16_choice_ms_per_4096_queries": tune, "d16_matrix_screen": bool(serve_flags & 0x8),
         "exact_vs_plain_bitwise": exact_vs_plain, "parity_vs_oracle_sample": parity, "oracle_checked_requests": checked,

@@ -117,3 +117,15 @@ def test_batch_item_layout_matches_header(tmp_path):
     out = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     assert out[0] == ctypes.sizeof(BatchItem)
     assert out[1:] == [getattr(BatchItem, f).offset for f in fields]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # The benchmark is a single-GPU workload: --gpus above 1 exits non-zero
+    # with a message, before touching any device.
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode != 0
+    assert "one GPU only" in (r.stdout + r.stderr)

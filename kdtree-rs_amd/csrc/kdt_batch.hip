@@ -1,14 +1,16 @@
-// kdt_batch.hip -- many trees, many query sets, one persistent launch.
+// kdt_batch.hip -- many trees, many query sets, few launches.
 //
 // No reference counterpart: the reference answers one query matrix against
 // one tree per call (Tree::query, src/query.rs:29-78).  Here a batch of
-// (tree, queries, k, outputs) items runs as one launch.  Every query of every
-// item gets a global number; one radix sort over the whole batch orders them
-// by (item, leaf reached by the near path) so that queries sharing a wavefront
-// share a tree and nodes; the groups of a persistent grid then take queries
-// from an atomic counter until the batch is drained and run the same per-query
-// descent as the single-tree kernels (knn_ww), so every result is identical
-// to a kdt_query_device call on the same item.
+// (tree, queries, k, outputs) items runs in one persistent launch per row
+// width.  Every query of those items gets a global number; one radix sort over
+// the batch orders them by (item, leaf reached by the near path) so that
+// queries sharing a wavefront share a tree and nodes; the groups of a
+// persistent grid then take queries from an atomic counter until the class is
+// drained and run the same per-query descent as the single-tree kernels
+// (knn_ww).  Large items, the matrix-core screen and the cases without a
+// compiled width run their own launch sequence.  Every result is identical to
+// a kdt_query_device call on the same item.
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
@@ -181,8 +183,8 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   base.allow_scratch = !no_order;
 
   // Validate; split items into width classes that run in shared launches and
-  // the rest (general p, other widths, k > 64, the matrix-core screen), which
-  // run one launch sequence each.
+  // the rest (general p, other widths, k > 64, the matrix-core screen, items of
+  // SINGLE_MIN queries or more), which run one launch sequence each.
   static const uint64_t widths[] = {1, 2, 3, 4, 8, 16};
   std::vector<uint64_t> cls_items[6];
   std::vector<uint64_t> single;

@@ -85,10 +85,10 @@ struct SArgs {
   uint32_t D, n, tiles_per_slice, n_tiles, slices;
 };
 
-#ifndef KDT_SCREEN_WAVES
-#define KDT_SCREEN_WAVES 8
-#endif
-constexpr int SW = KDT_SCREEN_WAVES;  // waves (32-query tiles) per workgroup sharing each point tile
+// Waves (32-query tiles) per workgroup sharing each point tile: 8 measured
+// fastest (4: each point tile read twice as often from L2; 16: fewer
+// workgroups than CUs can hold, a partial second round).
+constexpr int SW = 8;
 
 // Point tile t, 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS], KS / 2 uint4),
 // staged global -> registers -> LDS by the whole workgroup: thread u copies
@@ -132,11 +132,12 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, fl
 // instead of one per wave).  Lane l holds query j = l & 31, coordinates
 // [h KS, h KS + KS) with h = l >> 5, in both operands; instruction s of a
 // product contracts coordinates 8 s .. 8 s + 7 of each half.  Each FP32 value
-// v is split into bf16 halves hi + lo (lo = bf16(v - hi)) and q.p is formed
-// by three v_mfma_f32_32x32x16_bf16 per 16 coordinates: qh.ph + ql.ph + qh.pl
-// (bf16 x bf16 products are exact in FP32; the dropped ql.pl and the split
-// residual are in the error bound).  The accumulator holds D[i][j] for point
-// rows i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
+// v (of -2 q~ and of p~) is split into bf16 halves hi + lo (lo = bf16(v - hi))
+// and -2 q.p is formed by three v_mfma_f32_32x32x16_bf16 per 16 coordinates:
+// qh.ph + ql.ph + qh.pl (bf16 x bf16 products are exact in FP32; the dropped
+// ql.pl and the split residual are in the error bound); one more adds the
+// split norms.  Element r of the accumulator is then A for query j and point
+// row i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
 //
 // A value under the lane's threshold is not inserted at once: it is appended
 // to the lane's buffer of IB slots in LDS, and after every CHK elements, if any

@@ -493,6 +493,9 @@ __device__ __forceinline__ bool key_less(double da, uint32_t ia, double db, uint
 // Dynamic LDS: k x D doubles, the cell-bound axes of each certificate walk.
 __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   extern __shared__ double axes_all[];
+  constexpr uint32_t PATH_INNER = 32;       // inner nodes on a root-to-leaf path (the build checks depth <= 32)
+  __shared__ uint32_t s_chk[KP * PATH_INNER];  // per result lane: the children whose box gate is still to check
+  __shared__ uint32_t s_off[KP + 1];
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   const uint32_t D = a.D;
@@ -543,7 +546,13 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   const double kth = __shfl(d, (int)(a.k - 1));
   const double dcert = nres == a.k ? kth : a.cutoff;
 
+  // The certificate in two parts.  Each result lane walks its point's path
+  // with the plane gates and cell bounds inline (cheap) and records the
+  // children whose box gate must also hold; then the whole wave evaluates
+  // those box gates, spread over its 64 lanes (each one the same box_dist_d
+  // as before, so the same bits).
   bool ok = true;
+  uint32_t nchk = 0;
   if (lane < nres) {
     double* axes = axes_all + (size_t)lane * D;
     double cell = 0.0;
@@ -570,10 +579,14 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
       const uint32_t child = to_left ? node + 1 : nd.a;
       const double far_axis = reduce_d<M_L2>(fabs(diff), 2.0);
       const double far_dist = replace_axis_d<M_L2>(cell, axes[dim], far_axis);
-      if (nd.b & OBB_BIT) {  // descend_by_box, src/query.rs:296-327
-        ok = box_dist_d<M_L2, 0>(q, a.boxes + (size_t)child * 2 * D, D, 2.0) <= dcert;
-      } else if (is_far) {   // descend, src/query.rs:278-286
-        ok = far_dist <= dcert && box_dist_d<M_L2, 0>(q, a.boxes + (size_t)child * 2 * D, D, 2.0) <= dcert;
+      // descend_by_box (src/query.rs:296-327): the child's box gate; descend
+      // (src/query.rs:278-286): a far child's plane gate, then its box gate.
+      const bool box_gate = (nd.b & OBB_BIT) || is_far;
+      if (!(nd.b & OBB_BIT) && is_far) ok = far_dist <= dcert;
+      if (ok && box_gate) {
+        if (nchk < PATH_INNER) s_chk[lane * PATH_INNER + nchk] = child;
+        else ok = false;  // deeper than the build allows: leave the query to the traversal
+        ++nchk;
       }
       if (is_far) {          // enter_far, src/query.rs:329-341
         axes[dim] = far_axis;
@@ -589,6 +602,29 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
     }
   }
   if (__ballot(!ok)) {
+    if (lane == 0) a.flag[qi] = 1;
+    return;
+  }
+  // Box gates, all paths at once: offsets of each lane's list, then lane w
+  // takes checks w, w + 64, ...
+  const uint32_t mine = lane < nres ? nchk : 0u;
+  uint32_t incl = mine;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  if (lane < KP) s_off[lane] = incl - mine;
+  const uint32_t total = __shfl(incl, 63);
+  __syncthreads();
+  bool bad = false;
+  for (uint32_t w = lane; w < total; w += 64) {
+    uint32_t L = 0;
+    while (L + 1 < nres && s_off[L + 1] <= w) ++L;
+    const uint32_t child = s_chk[L * PATH_INNER + (w - s_off[L])];
+    bad |= !(box_dist_d<M_L2, 0>(q, a.boxes + (size_t)child * 2 * D, D, 2.0) <= dcert);
+  }
+  if (__ballot(bad)) {
     if (lane == 0) a.flag[qi] = 1;
     return;
   }

@@ -51,7 +51,7 @@ enum kdt_error {
 /* Query flags. */
 #define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 leaf screen with a proven error margin,
                                       FP64 recheck: identical results, fewer bytes.
-                                      For 32 <= ndim <= 64, p = 2, eps = 0, k <= 16:
+                                      For 32 <= ndim <= 64, p = 2, eps = 0, k <= 32:
                                       matrix-core screen over every point, FP64
                                       recheck and a visit certificate; queries the
                                       certificate does not cover run the traversal */

@@ -71,7 +71,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr uint32_t CAND_CAP = 64;  // candidates per query (one per lane of the recheck)
-constexpr int KP = 16;             // largest k on this path
+constexpr int KP = 16;             // entries per lane list
+constexpr uint32_t KMAX = 32;      // largest k on this path (k > KP takes more point slices, below)
 constexpr int LISTS_PER_LANE = 4;  // k_screen_select reads 64 x 4 per-lane lists at most
 constexpr uint32_t MAX_SLICES = 32 * LISTS_PER_LANE;  // two lists (lane halves) per slice
 
@@ -493,9 +494,9 @@ __device__ __forceinline__ bool key_less(double da, uint32_t ia, double db, uint
 // Dynamic LDS: k x D doubles, the cell-bound axes of each certificate walk.
 __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   extern __shared__ double axes_all[];
-  constexpr uint32_t PATH_INNER = 32;       // inner nodes on a root-to-leaf path (the build checks depth <= 32)
-  __shared__ uint32_t s_chk[KP * PATH_INNER];  // per result lane: the children whose box gate is still to check
-  __shared__ uint32_t s_off[KP + 1];
+  constexpr uint32_t PATH_INNER = 32;         // inner nodes on a root-to-leaf path (the build checks depth <= 32)
+  __shared__ uint32_t s_chk[KMAX * PATH_INNER];  // per result lane: the children whose box gate is still to check
+  __shared__ uint32_t s_off[KMAX + 1];
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   const uint32_t D = a.D;
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
     const uint32_t v = __shfl_up(incl, o);
     if (lane >= o) incl += v;
   }
-  if (lane < KP) s_off[lane] = incl - mine;
+  if (lane < KMAX) s_off[lane] = incl - mine;
   const uint32_t total = __shfl(incl, 63);
   __syncthreads();
   bool bad = false;
@@ -643,7 +644,7 @@ int dev_alloc(T** p, size_t count, hipStream_t s) {
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.allow_scratch && qp.metric == M_L2 && qp.eps_factor == 1.0 &&
-         qp.k <= (uint64_t)KP && t.rows_bf &&
+         qp.k <= (uint64_t)KMAX && t.rows_bf &&
          qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64;
 }
 
@@ -654,7 +655,11 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
   // Slices of the points per query tile: enough waves to fill the device,
   // at least 8 tiles per slice.
+  // For k > KP, at least k / 4 slices: a query's top k then spread over at
+  // least k / 2 lists of KP, about two per list, so a list rarely fills below
+  // tau_A (the completeness check flags the query when one does).
   uint64_t slices = std::max<uint64_t>(1, (4096 + qtiles - 1) / qtiles);
+  if (k > (uint32_t)KP) slices = std::max<uint64_t>(slices, (k + 3) / 4);
   slices = std::min<uint64_t>({slices, MAX_SLICES, std::max<uint32_t>(1, n_tiles / 8)});
   const uint32_t per = (uint32_t)((n_tiles + slices - 1) / slices);
   slices = (n_tiles + per - 1) / per;

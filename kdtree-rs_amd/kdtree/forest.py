@@ -128,7 +128,7 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
                    reps: int = 3) -> DeviceTree:
     """Build with each leafsize, time `reps` queries of the sample with each
     group width (lanes per query that hold k), keep the fastest pair; then,
-    where the matrix-core screen applies (16 <= d <= 64, k <= 16, for queries
+    where the matrix-core screen applies (16 <= d <= 64, k <= 32, for queries
     with p = 2 and eps = 0), time it too and keep it in `flags` if faster.
 
     The leafsize changes the tree, and for 4- and 8-dimensional rows the
@@ -163,7 +163,7 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
         if not keep:
             tree.free()
     best.group = best_g
-    if 16 <= best.d <= 64 and k <= 16:
+    if 16 <= best.d <= 64 and k <= 32:
         sflags = KDT_QUERY_FP32_SCREEN | KDT_QUERY_MATRIX_SCREEN
         out_d, out_i = query_device(best, q, k, flags=KDT_QUERY_ASSUME_FINITE | sflags)
         torch.cuda.synchronize()

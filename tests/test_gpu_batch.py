@@ -88,7 +88,7 @@ def test_item_screen_flags_match_oracle(forest):
     saved = [t.flags for t in trees]
     forced = 0
     for tree, _, k, _, _ in batch.items:
-        if 16 <= tree.d <= 64 and k <= 16:
+        if 16 <= tree.d <= 64 and k <= 32:
             tree.flags = sflags
             forced += 1
     assert forced > 0, "no item the matrix-core screen applies to"

@@ -1,4 +1,4 @@
-"""Matrix-core screen for 32 <= ndim <= 64 (p = 2, eps = 0, k <= 16).
+"""Matrix-core screen for 32 <= ndim <= 64 (p = 2, eps = 0, k <= 32).
 
 Every result must equal the oracle's (the reference's descent restated) bit
 for bit, whichever way a query is answered: by the screen + FP64 recheck +
@@ -53,6 +53,11 @@ CASES = {
     "cutoff-5000x64-k16": (lambda: f32_normal(17, 5000, 64), lambda: f32_normal(18, 200, 64), 16, 32, 10.0, (0.95, 1.0)),
     "far-queries-5000x32-k4": (lambda: f32_normal(19, 5000, 32), lambda: 50.0 + f32_normal(20, 100, 32), 4, 32,
                                None, (0.0, 1.0)),
+    # k above the 16-entry lane lists: more point slices per query
+    "normal-f32-20000x64-k32": (lambda: f32_normal(31, 20_000, 64), lambda: f32_normal(32, 600, 64), 32, 32, None,
+                                (0.9, 1.0)),
+    "normal-6000x40-k20-cutoff": (lambda: np.random.default_rng(33).normal(size=(6000, 40)),
+                                  lambda: np.random.default_rng(34).normal(size=(300, 40)), 20, 16, 9.0, (0.0, 1.0)),
 }
 
 
@@ -99,6 +104,7 @@ NARROW = {
                                 lambda: np.random.default_rng(26).uniform(size=(300, 16)), 16, 8),
     "normal-4000x24-k3": (lambda: np.random.default_rng(27).normal(size=(4000, 24)),
                           lambda: np.random.default_rng(28).normal(size=(129, 24)), 3, 32),
+    "normal-f32-12000x16-k32": (lambda: f32_normal(35, 12_000, 16), lambda: f32_normal(36, 500, 16), 32, 32),
 }
 
 
@@ -123,19 +129,21 @@ def test_matrix_screen_narrow_rows(case):
     np.testing.assert_array_equal(ti, oi)
 
 
-def test_screen_large_batch_device():
-    """2^14 queries on the device, many point slices per query tile."""
+@pytest.mark.parametrize("k", [10, 32])
+def test_screen_large_batch_device(k):
+    """2^14 queries on the device: few point slices per query tile (k = 32
+    takes at least 8, so that a query's top k spreads over its lists)."""
     import torch
 
     from kdtree import forest as F
 
     data = f32_normal(21, 50_000, 64)
     q = f32_normal(22, 1 << 14, 64)
-    od, oi = OracleTree(data, 32).query(q, k=10, threads=8)
+    od, oi = OracleTree(data, 32).query(q, k=k, threads=8)
     tree = F.build(data, 32)
     try:
         qd = torch.from_numpy(q).cuda()
-        gd, gi = F.query_device(tree, qd, 10, flags=F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN)
+        gd, gi = F.query_device(tree, qd, k, flags=F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(gi.cpu().numpy(), oi)
         assert gd.cpu().numpy().tobytes() == od.tobytes()

@@ -680,10 +680,12 @@ __global__ void k_validate_nodes(const DevNode* __restrict__ nodes, uint64_t n_n
 // Matrix-core screen rows: tree-order rows rounded to f32 (v), zero-padded to
 // `stride` columns and n_pad rows, each stored as two bf16 halves,
 // hi = bf16(v) and lo = bf16(v - hi): row r is [hi x stride | lo x stride] in
-// rows_bf.  pnorm = |v|^2 (+inf on padding rows); *pmax_bits = max over rows
-// of |v| rounded up, as f32 bits.
+// rows_bf.  pnorm_bf = pn = fl32(|v|^2) (+inf on padding rows) split the same
+// way, packed as bf16(pn) | bf16(pn - hi) << 16 (the residual 0 for +inf): the
+// word the screen's norm block takes as its operand.  *pmax_bits = max over
+// rows of |v| rounded up, as f32 bits.
 __global__ void k_rows_split(const double* __restrict__ rows, uint64_t n, uint32_t D, uint32_t stride,
-                             uint64_t n_pad, uint16_t* __restrict__ rows_bf, float* __restrict__ pnorm,
+                             uint64_t n_pad, uint16_t* __restrict__ rows_bf, uint32_t* __restrict__ pnorm_bf,
                              uint32_t* pmax_bits) {
   float mx = 0.0f;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_pad;
@@ -697,7 +699,10 @@ __global__ void k_rows_split(const double* __restrict__ rows, uint64_t n, uint32
       rows_bf[r * 2 * stride + stride + c] = __builtin_bit_cast(uint16_t, lo);
       s += (double)v * (double)v;
     }
-    pnorm[r] = r < n ? (float)s : INFINITY;
+    const float pn = r < n ? (float)s : INFINITY;
+    const __bf16 ph = (__bf16)pn;
+    const __bf16 pl = (__bf16)(__builtin_isinf(pn) ? 0.0f : pn - (float)ph);
+    pnorm_bf[r] = (uint32_t)__builtin_bit_cast(uint16_t, ph) | ((uint32_t)__builtin_bit_cast(uint16_t, pl) << 16);
     if (r < n) {
       const double nr = sqrt(s);
       float f = (float)nr;
@@ -817,7 +822,8 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<LvlNode> d_lvl;
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
-  DevBuf<float> pts32, pnorm;
+  DevBuf<float> pts32;
+  DevBuf<uint32_t> pnorm_bf;
   DevBuf<uint16_t> rows_bf;
   DevBuf<uint32_t> idx, tidx, alist, blist, cnt, off, hist, node_len, d_leaves;
   DevBuf<uint64_t> ukey;
@@ -837,7 +843,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   const uint64_t n_pad = (n + 31) / 32 * 32;
   if (wide) {
     KDT_HIP_CHECK(rows_bf.alloc(n_pad * 2 * stride32));
-    KDT_HIP_CHECK(pnorm.alloc(n_pad));
+    KDT_HIP_CHECK(pnorm_bf.alloc(n_pad));
   }
   KDT_HIP_CHECK(idx.alloc(n));
   KDT_HIP_CHECK(tidx.alloc(n));
@@ -926,7 +932,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   k_leaf_buckets<<<(uint32_t)(leaves.size() / 2), 64, 0, s>>>(rows.p, Du, d_leaves.p, pts.p, pts32.p);
   if (wide) {
     k_rows_split<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
-        rows.p, n, Du, stride32, n_pad, rows_bf.p, pnorm.p, reinterpret_cast<uint32_t*>(err.p) + 2);
+        rows.p, n, Du, stride32, n_pad, rows_bf.p, pnorm_bf.p, reinterpret_cast<uint32_t*>(err.p) + 2);
   }
   k_validate_nodes<<<(uint32_t)std::min<uint64_t>((t.n_nodes + 255) / 256, 4096), 256, 0, s>>>(
       nodes.p, t.n_nodes, n, Du, err.p);
@@ -964,7 +970,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
     std::memcpy(&pm, &h_pmax, 4);
     t.pmax = (double)pm * (1.0 + 1e-6);
     t.rows_bf = rows_bf.release();
-    t.pnorm = pnorm.release();
+    t.pnorm_bf = pnorm_bf.release();
   }
   return 0;
 }

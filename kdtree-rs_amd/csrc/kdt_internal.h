@@ -14,7 +14,7 @@
 //                                      16, 32 or 64 and to n_pad = 32 * ceil(n / 32)
 //                                      rows, as bf16 hi = bf16(v) then lo =
 //                                      bf16(v - hi), for the matrix-core screen
-//   pnorm  f32[n_pad]                  |v|^2 (+inf on padding rows)
+//   pnorm_bf u32[n_pad]                fl32(|v|^2) (+inf on padding rows) as a packed bf16 pair (hi | lo << 16)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -52,7 +52,7 @@ struct Tree {
   double* rows = nullptr;
   double max_abs = 0.0;  // max |coordinate| over the data (root box)
   uint16_t* rows_bf = nullptr;  // bf16 bit patterns
-  float* pnorm = nullptr;
+  uint32_t* pnorm_bf = nullptr;
   double pmax = 0.0;     // upper bound on max |row32|
 };
 

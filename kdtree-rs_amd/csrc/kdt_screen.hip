@@ -78,7 +78,7 @@ constexpr uint32_t MAX_SLICES = 32 * LISTS_PER_LANE;  // two lists (lane halves)
 
 struct SArgs {
   const uint16_t* __restrict__ rows_bf;  // [n_pad][hi 2 KS | lo 2 KS] bf16, tree order, zero-padded
-  const float* __restrict__ pnorm;   // [n_pad] |p~|^2, +inf on padding rows
+  const uint32_t* __restrict__ pnorm_bf;  // [n_pad] |p~|^2 (+inf on padding rows), packed bf16 hi | lo << 16
   const double* __restrict__ q;      // [nq][D]
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
   uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
@@ -93,7 +93,7 @@ constexpr int SW = 8;
 
 // Point tile t, 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS], KS / 2 uint4),
 // staged global -> registers -> LDS by the whole workgroup: thread u copies
-// uint4 u, u + 64 SW, ... of the tile (coalesced), and threads 0..31 its |p~|^2.
+// uint4 u, u + 64 SW, ... of the tile (coalesced), and threads 0..31 its split |p~|^2.
 // (Nothing is read past the tile: TU and 32 bound every index.)
 template <int KS>
 struct Stage {
@@ -108,18 +108,18 @@ struct Stage {
 // unconditional, at clamped (in-bounds) indices; stage_store keeps u < TU.
 template <int KS>
 __device__ __forceinline__ void stage_load(const SArgs& a, uint32_t t, uint32_t tid, uint4& v0, uint4& v1,
-                                           float& pn) {
+                                           uint32_t& pn) {
   const uint4* src = reinterpret_cast<const uint4*>(a.rows_bf) + (size_t)t * Stage<KS>::TU;
   v0 = src[min(tid, (uint32_t)Stage<KS>::TU - 1)];
   if (Stage<KS>::PT > 1) v1 = src[min(tid + 64 * SW, (uint32_t)Stage<KS>::TU - 1)];
-  pn = a.pnorm[(size_t)t * 32 + (tid & 31)];
+  pn = a.pnorm_bf[(size_t)t * 32 + (tid & 31)];
 }
 
 // LDS rows are padded by one uint4 (stride KS / 2 + 1 uint4), so the 32 lanes
 // of a half-wave reading 32 different rows hit distinct banks.
 template <int KS>
-__device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, float pnv, uint32_t tid, uint4* rows,
-                                            float* pn) {
+__device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, uint32_t pnv, uint32_t tid,
+                                            uint4* rows, uint32_t* pn) {
   constexpr int RU = Stage<KS>::RU;
   if (tid < (uint32_t)Stage<KS>::TU) rows[(tid / RU) * (RU + 1) + tid % RU] = v0;
   const uint32_t u = tid + 64 * SW;
@@ -196,16 +196,12 @@ struct Screen {
     cnt = 0;
   }
 
-  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const float* pns) {
+  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint32_t* pns) {
     const uint4* rb = rows + j * RS + h * NS;
     // Norm block, A side: (1, 1, pn_hi, pn_lo, 0...) of row j in half 0, so one
     // more instruction adds qn + pn (four exact terms) to every element and the
     // accumulator ends as A itself.  Padding rows: pn = +inf, pn_lo = 0, A = +inf.
-    const float pnj = pns[j];
-    const __bf16 pnh = (__bf16)pnj;
-    const __bf16 pnl = (__bf16)(__builtin_isinf(pnj) ? 0.0f : pnj - (float)pnh);
-    const uint32_t pw = (uint32_t)__builtin_bit_cast(uint16_t, pnh) |
-                        ((uint32_t)__builtin_bit_cast(uint16_t, pnl) << 16);
+    const uint32_t pw = pns[j];  // split in the build
     const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u);
     // The point operands of 16 coordinates are read one group ahead of their
     // three products (left to itself the scheduler reads all 2 NS operands
@@ -256,7 +252,7 @@ template <int KS>
 __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
   constexpr int RS = KS / 2 + 1;
   __shared__ uint4 s_rows[2][32 * RS];
-  __shared__ __align__(16) float s_pn[2][32];
+  __shared__ __align__(16) uint32_t s_pn[2][32];
   __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
@@ -317,7 +313,7 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
   } while (0)
   if (t0 < t1) {  // uniform over the workgroup
     uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
-    float xp = 0.0f, yp = 0.0f;
+    uint32_t xp = 0, yp = 0;
     stage_load<KS>(a, t0, tid, x0, x1, xp);
     stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
     stage_load<KS>(a, min(t0 + 1, t1 - 1), tid, x0, x1, xp);
@@ -684,7 +680,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   }
   SArgs sa{};
   sa.rows_bf = t.rows_bf;
-  sa.pnorm = t.pnorm;
+  sa.pnorm_bf = t.pnorm_bf;
   sa.q = trav.q;
   sa.part_a = part_a;
   sa.part_i = part_i;

@@ -18,7 +18,8 @@
 //                                  (the reference API never exposes it)
 //   query indices ................ SciPy fixtures, brute force, lattice ties
 //   distances (L1/L2/Linf) ....... SciPy fixtures (1e-12) + per-width
-//                                  summation known-answer vectors
+//                                  summation closed forms (terms 1 and
+//                                  2^-53: a wrong pairing flips the last bit)
 //   distances (general p) ........ SciPy fixtures (1e-12); bits of powf are
 //                                  platform libm, so bit-level parity unpinned
 

@@ -13,6 +13,7 @@ import pytest
 
 from kdtree import KDTree
 from oracle import OracleTree
+from summation_cases import ORDER_CASES
 
 pytestmark = pytest.mark.gpu
 
@@ -110,6 +111,22 @@ def test_leafsizes_partial_chunks(leafsize, dims):
     data = np.random.default_rng(20 + leafsize).normal(size=(2500, dims))
     queries = np.random.default_rng(21).normal(size=(100, dims))
     _query_case(data, queries, k=5, leafsize=leafsize)
+
+
+@pytest.mark.parametrize("case", ORDER_CASES, ids=[c[0] for c in ORDER_CASES])
+def test_summation_order_closed_forms(case):
+    # The oracle's closed-form order vectors through the device kernels: one
+    # leaf of `length` rows (length <= leafsize, so rows keep their input
+    # order), the vector at row `offset`, every other row far away; L1, so the
+    # returned distance is the sum of the terms itself (1 or 1 + 2^-52).
+    _, d, v, off, length, want, _ = case
+    data = 1000.0 + np.arange(length * d, dtype=np.float64).reshape(length, d)
+    data[off] = v
+    tree = KDTree(data, leafsize=32)
+    assert tree._export()[1][off] == off
+    gd, gi = tree.query(np.zeros((1, d)), k=length, p=1.0)
+    assert gi[0, 0] == off
+    assert gd[0, 0] == want, f"{gd[0, 0]!r} != {want!r}"
 
 
 @pytest.mark.parametrize("p", [1.0, 2.0, 3.0, np.inf])

@@ -5,6 +5,8 @@ Each check is chosen so a plausible restatement mistake fails it:
     67-89): indices exactly, distances at 1e-12 -- a wrong metric, a dropped
     term, a wrong prune all change indices.
   * Closed forms (one non-zero axis, so every summation order agrees).
+  * Closed forms that fix each row width's summation order: terms 1 and
+    2^-53, where every plausible wrong pairing changes the last bit.
   * Brute force on integer lattices, where every distance is exact in FP64 and
     ties are everywhere: pins the (distance, index) tie rule, padding and
     max_distance independently of summation order.
@@ -297,6 +299,22 @@ def test_point_dist_width_recipes_are_plausible():
         want = ((q - pt) ** 2).sum()
         for off, length in [(0, 32), (31, 33)]:
             assert abs(point_dist(q, pt, 2.0, off, length) - want) <= 1e-13 * max(1.0, want)
+
+
+from summation_cases import ONE, ONE_UP, ORDER_CASES, terms  # noqa: E402
+
+
+@pytest.mark.parametrize("case", ORDER_CASES, ids=[c[0] for c in ORDER_CASES])
+def test_point_dist_summation_order_closed_forms(case):
+    _, d, v, off, length, want, _mistake = case
+    got = point_dist(np.zeros(d), v, 1.0, off, length)
+    assert got == want, f"{got!r} != {want!r}"
+    # L2 with squares of powers of two: terms 1 and four of 2^-54 (two pairs
+    # make 2^-52) where the L1 case relies on a pair of T.
+    if d == 8 and off == 0:
+        sq = terms(8, a0=1.0, a4=2.0**-27, a5=2.0**-27, a6=2.0**-27, a7=2.0**-27)
+        assert point_dist(np.zeros(8), sq, 2.0, 0, 8) == ONE_UP
+        assert point_dist(np.zeros(8), sq, 2.0, 8, 9) == ONE
 
 
 # ---------------------------------------------------------- property test

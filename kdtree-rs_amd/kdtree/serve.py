@@ -74,11 +74,11 @@ class Server:
         an upper bound of the matrix-core screen's scratch (two 16-entry lists
         of 8-byte entries per query and point slice, with the launch's slice
         rule: enough slices for 4096 query-tile waves, at least k / 4 for
-        k > 16, at most 128; then 64 candidates, a count and a flag per query)."""
+        k > 16, at most 512; then 64 candidates, a count and a flag per query)."""
         m, d = r.queries.shape
         qtiles = -(-m // 32)
         slices = max(-(-4096 // qtiles), -(-r.k // 4) if r.k > 16 else 1)
-        slices = min(slices, 128)
+        slices = min(slices, 512)
         return m * d * 8 + m * r.k * 16 + m * (256 * slices + 261)
 
     def _dispatch(self, reqs, stream) -> _Launch:

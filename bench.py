@@ -144,7 +144,9 @@ def main():
     from oracle import OracleTree
 
     sample = 2000
+    t0 = time.perf_counter()
     otree = OracleTree(data_h, ls)
+    cpu_build_ms = (time.perf_counter() - t0) * 1e3  # the oracle's build: one thread
     od, oi, ctr = otree.query(q_h[:10_000], k=k, counters=True)
     gd = out_d[:sample].cpu().numpy()
     gi = out_i[:sample].cpu().numpy()
@@ -200,6 +202,9 @@ def main():
         "latency_ms": {"p50": float(np.percentile(kernel_ms, 50)), "p99": float(np.percentile(kernel_ms, 99)),
                        "what": "one full 2^20-query batch"},
         "build_ms": build_ms,
+        "build_vs_cpu": {"gpu_ms": build_ms, "cpu_ms": cpu_build_ms, "cpu_threads": 1, "kind": "port",
+                         "what": "build of the 2^20-point tree: oracle on one host thread (the build parity "
+                                 "tests check that both trees are identical); GPU time includes its allocations"},
         "parity_sample_bit_exact": parity,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -254,7 +259,9 @@ def run_c3(args):
     d64, d32 = od64.cpu().numpy(), od32.cpu().numpy()
     rel = float(np.max(np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-300)))
     sample = 2000
+    t0 = time.perf_counter()
     otree = OracleTree(data_h, ls)
+    cpu_build_ms = (time.perf_counter() - t0) * 1e3  # the oracle's build: one thread
     od, oi = otree.query(q_h[:sample], k=k)
     parity = bool(np.array_equal(oi, oi64[:sample].cpu().numpy()) and od.tobytes() == d64[:sample].tobytes())
     cpu = None

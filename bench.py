@@ -183,6 +183,20 @@ def main():
             "one_core": {"value": one_core, "sample": f"{m1} queries, 1 thread"},
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
         }
+        try:  # secondary: SciPy's cKDTree (the reference's own tests compare against SciPy)
+            import scipy
+            from scipy.spatial import cKDTree
+        except ImportError:
+            scipy = None
+        if scipy is not None:
+            t0 = time.perf_counter()
+            stree = cKDTree(data_h, leafsize=ls)
+            sbuild = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            stree.query(q_h[:m], k=k, workers=threads)
+            cpu["scipy_ckdtree"] = {"value": m / (time.perf_counter() - t0), "unit": "queries/s",
+                                    "workers": threads, "build_ms": sbuild * 1e3, "version": scipy.__version__,
+                                    "sample": f"the same {m} queries and points, leafsize {ls}"}
 
     line = {
         "metric": "k-NN queries/s (exact indices), plain GPU kernel",

@@ -9,13 +9,16 @@
 // persistent grid then take queries from an atomic counter until the class is
 // drained and run the same per-query descent as the single-tree kernels
 // (knn_ww).  Large items, the matrix-core screen and the cases without a
-// compiled width run their own launch sequence.  Every result is identical to
-// a kdt_query_device call on the same item.
+// compiled width run their own launch sequence, spread over the caller's
+// stream and side streams forked from it.  Every result is identical to a
+// kdt_query_device call on the same item.
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/kdtree_hip.h"
@@ -159,6 +162,43 @@ void item_flags(const kdt_batch_item& it, QueryParams& qp) {
   qp.fp32_screen = qp.fp32_screen || (it.flags & KDT_QUERY_FP32_SCREEN) != 0;
   qp.matrix_screen = qp.matrix_screen || (it.flags & KDT_QUERY_MATRIX_SCREEN) != 0;
 }
+
+// Side streams for the items that run their own launch sequence: one item of
+// 8192 queries fills about a quarter of the device, so consecutive such items
+// on one stream leave it idle.  The call forks the side streams from the
+// caller's stream (an event), spreads those items over the caller's stream and
+// them, largest first, and joins them back (one event each) after the shared
+// launches, which stay on the caller's stream and overlap with them.  Created
+// once per device, default priority; `lock` keeps one call at a time on a
+// device's side streams and events, and `rot` varies which side stream a
+// call's second item takes, so that calls from several caller streams spread.
+constexpr int N_SIDE = 4;
+struct SideStreams {
+  hipStream_t s[N_SIDE] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t done[N_SIDE] = {};
+  std::mutex lock;
+  uint32_t rot = 0;
+  bool ok = false;
+};
+
+SideStreams* side_streams() {
+  static std::mutex m;
+  static std::map<int, SideStreams*> per_device;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> g(m);
+  SideStreams*& ss = per_device[dev];
+  if (!ss) {
+    ss = new SideStreams;  // kept until the process ends, like the device context
+    bool ok = hipEventCreateWithFlags(&ss->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; i < N_SIDE && ok; ++i)
+      ok = hipStreamCreateWithFlags(&ss->s[i], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&ss->done[i], hipEventDisableTiming) == hipSuccess;
+    ss->ok = ok;
+  }
+  return ss->ok ? ss : nullptr;
+}
 }  // namespace
 }  // namespace kdt
 
@@ -215,6 +255,34 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     else cls_items[c].push_back(i);
   }
 
+  // Several own-sequence items are spread over the caller's stream and the
+  // side streams, unless the call is a pure enqueue (KDT_BATCH_NO_ORDER, e.g.
+  // under graph capture): then every launch stays on the caller's stream.
+  SideStreams* side = (!no_order && single.size() > 1) ? side_streams() : nullptr;
+  std::unique_lock<std::mutex> side_lock;
+  if (side) {
+    side_lock = std::unique_lock<std::mutex>(side->lock);
+    bool ok = hipEventRecord(side->fork, s) == hipSuccess;
+    for (int j = 0; j < N_SIDE && ok; ++j) ok = hipStreamWaitEvent(side->s[j], side->fork, 0) == hipSuccess;
+    if (!ok) return batch_fail(KDT_ERR_DEVICE, "batch: side stream fork failed");
+  }
+  // Joins the side streams back into the caller's stream (also on errors, so
+  // that nothing of this call is left unordered behind the caller's stream).
+  auto join = [&]() -> bool {
+    if (!side) return true;
+    bool ok = true;
+    for (int j = 0; j < N_SIDE; ++j)
+      ok = hipEventRecord(side->done[j], side->s[j]) == hipSuccess &&
+           hipStreamWaitEvent(s, side->done[j], 0) == hipSuccess && ok;
+    return ok;
+  };
+  uint32_t r0 = 0;
+  if (side) {
+    r0 = side->rot++;
+    std::stable_sort(single.begin(), single.end(),
+                     [&](uint64_t x, uint64_t y) { return items[x].n_queries > items[y].n_queries; });
+  }
+  uint64_t next_side = 0;
   for (uint64_t i : single) {
     const kdt_batch_item& it = items[i];
     QueryParams qp = base;
@@ -223,8 +291,16 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     item_flags(it, qp);
     qp.allow_order = !no_order;
     if (it.group) qp.group = (int)it.group;
-    rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, s, msg);
-    if (rc) return batch_fail(rc, msg);
+    hipStream_t si = s;
+    if (side) {
+      const uint64_t j = next_side++ % (N_SIDE + 1);
+      if (j) si = side->s[(r0 + j - 1) % N_SIDE];
+    }
+    rc = launch_query(it.tree->t, qp, it.queries, it.out_dist, it.out_idx, si, msg);
+    if (rc) {
+      join();
+      return batch_fail(rc, msg);
+    }
   }
 
   // Shared launches: items numbered class by class, so each class is one
@@ -252,8 +328,11 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
     }
     count[c] = total - first[c];
   }
-  if (h.empty()) return KDT_OK;
-  if (total > 0xFFFFFFFFull) return batch_fail(KDT_ERR_ARGUMENT, "batch exceeds 2^32 queries");
+  if (h.empty()) return join() ? KDT_OK : batch_fail(KDT_ERR_DEVICE, "batch: side stream join failed");
+  if (total > 0xFFFFFFFFull) {
+    join();
+    return batch_fail(KDT_ERR_ARGUMENT, "batch exceeds 2^32 queries");
+  }
 
   BItem* d_items = nullptr;
   unsigned long long* counters = nullptr;
@@ -318,6 +397,8 @@ extern "C" int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_it
   if (order) (void)hipFreeAsync(order, s);
   if (counters) (void)hipFreeAsync(counters, s);
   if (d_items) (void)hipFreeAsync(d_items, s);
+  const bool joined = join();
   if (e != hipSuccess) return batch_fail(KDT_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+  if (!joined) return batch_fail(KDT_ERR_DEVICE, "batch: side stream join failed");
   return KDT_OK;
 }

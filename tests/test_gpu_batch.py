@@ -146,3 +146,47 @@ def test_batch_metrics_eps_cutoff_match_oracle(p, eps, max_distance, per_tree):
     finally:
         for tree, _, _ in items:
             tree.free()
+
+
+def test_side_streams_join_into_caller_stream():
+    # Items of 8192+ queries run their own launch sequences, spread over the
+    # caller's stream and side streams forked from it; waiting on the caller's
+    # stream alone (no device-wide synchronisation) must see every result.
+    import torch
+
+    from kdtree import forest as F
+
+    rng = np.random.default_rng(90)
+    items, host = [], []
+    for t in range(6):
+        data = rng.uniform(size=(40_000, 3))
+        q = rng.uniform(size=(12_000 + 1_000 * t, 3))
+        items.append((F.build(data, 32), torch.from_numpy(q).cuda(), 8))
+        host.append((data, q))
+    batch = F.Batch(items)
+    want = [OracleTree(data, 32).query(q, k=8, threads=8) for data, q in host]
+    torch.cuda.synchronize()
+    s = torch.cuda.Stream()
+    try:
+        for mode in ("persistent", "per_tree"):
+            print(f"mode {mode}", flush=True)
+            with torch.cuda.stream(s):
+                for _, _, _, od, oi in batch.items:
+                    od.fill_(-1.0)
+                    oi.fill_(-1)
+                batch.run(mode, stream=s)
+                got = []
+                for _, _, _, od, oi in batch.items:
+                    hd = torch.empty(od.shape, dtype=od.dtype, pin_memory=True)
+                    hi = torch.empty(oi.shape, dtype=oi.dtype, pin_memory=True)
+                    hd.copy_(od, non_blocking=True)
+                    hi.copy_(oi, non_blocking=True)
+                    got.append((hd, hi))
+            s.synchronize()
+            for j, ((hd, hi), (od, oi)) in enumerate(zip(got, want)):
+                np.testing.assert_array_equal(hi.numpy(), oi, err_msg=f"{mode} item {j}")
+                assert hd.numpy().tobytes() == od.tobytes(), f"{mode} item {j} distances"
+    finally:
+        torch.cuda.synchronize()
+        for tree, _, _ in items:
+            tree.free()

@@ -5,10 +5,15 @@
 // src/build.rs:19-82), Tree::query (src/query.rs:29-78), Tree::original_data
 // (src/tree.rs:60-68) and KDTreeError's Display (src/error.rs:18-51).
 
+#include <algorithm>
+#include <atomic>
 #include <charconv>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/kdtree_hip.h"
@@ -45,6 +50,115 @@ struct DevMem {
     return hipMallocAsync((void**)&p, std::max<size_t>(n, 1) * sizeof(T), st);
   }
 };
+
+// Copies between device memory and the caller's pageable host arrays.  A
+// pageable copy goes through the runtime's own staging at a fraction of the
+// link rate, so large copies are staged here instead: STAGE_WORKERS host
+// threads each take STAGE_BYTES chunks and move them through their own pinned
+// buffer on their own stream (device -> pinned -> caller's array, or the
+// reverse), so that one worker's DMA overlaps the others' host copies.  The
+// pinned buffers (hipHostMalloc) and streams (default priority) are made once
+// per device and kept until the process ends; one call at a time uses them.
+constexpr size_t STAGE_BYTES = 8u << 20;
+constexpr int STAGE_WORKERS = 8;
+
+struct Staging {
+  std::mutex lock;
+  void* buf[STAGE_WORKERS] = {};
+  hipStream_t ws[STAGE_WORKERS] = {};
+  bool ok = false;
+};
+
+Staging* staging_for(int dev) {
+  static std::mutex m;
+  static std::map<int, Staging*> per_device;
+  std::lock_guard<std::mutex> g(m);
+  Staging*& st = per_device[dev];
+  if (!st) {
+    st = new Staging;
+    bool ok = true;
+    for (int w = 0; w < STAGE_WORKERS && ok; ++w)
+      ok = hipHostMalloc(&st->buf[w], STAGE_BYTES, hipHostMallocDefault) == hipSuccess &&
+           hipStreamCreateWithFlags(&st->ws[w], hipStreamNonBlocking) == hipSuccess;
+    st->ok = ok;
+  }
+  return st;
+}
+
+struct HostSeg {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+
+// Copies every segment (device -> host when `to_host`, else host -> device).
+// `s` is the stream that wrote (or will read) the device side: the copy is
+// ordered after its work so far, and small copies are simply enqueued on it;
+// staged copies have finished when the call returns.  Returns the first HIP error.
+hipError_t staged_copy(const std::vector<HostSeg>& segs, bool to_host, hipStream_t s) {
+  const hipMemcpyKind kind = to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice;
+  size_t total = 0;
+  for (const HostSeg& g : segs) total += g.bytes;
+  if (total < 2 * STAGE_BYTES) {
+    for (const HostSeg& g : segs) {
+      const hipError_t e = hipMemcpyAsync(g.dst, g.src, g.bytes, kind, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  Staging& st = *staging_for(dev);
+  std::lock_guard<std::mutex> guard(st.lock);
+  e = hipStreamSynchronize(s);  // results complete, or device buffers free to overwrite
+  if (e != hipSuccess) return e;
+  if (!st.ok) {  // no staging memory: the runtime's own path
+    for (const HostSeg& g : segs) {
+      e = hipMemcpy(g.dst, g.src, g.bytes, kind);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  std::vector<std::pair<size_t, size_t>> chunks;  // (segment, offset)
+  for (size_t i = 0; i < segs.size(); ++i)
+    for (size_t off = 0; off < segs[i].bytes; off += STAGE_BYTES) chunks.push_back({i, off});
+  std::atomic<size_t> next{0};
+  std::atomic<int> err{(int)hipSuccess};
+  auto worker = [&](int w) {
+    if (hipSetDevice(dev) != hipSuccess) {
+      err = (int)hipErrorInvalidDevice;
+      return;
+    }
+    for (size_t c = next++; c < chunks.size() && err == (int)hipSuccess; c = next++) {
+      const HostSeg& g = segs[chunks[c].first];
+      const size_t off = chunks[c].second, len = std::min(STAGE_BYTES, g.bytes - off);
+      hipError_t we;
+      if (to_host) {
+        we = hipMemcpyAsync(st.buf[w], (const char*)g.src + off, len, kind, st.ws[w]);
+        if (we == hipSuccess) we = hipStreamSynchronize(st.ws[w]);
+        if (we == hipSuccess) std::memcpy((char*)g.dst + off, st.buf[w], len);
+      } else {
+        std::memcpy(st.buf[w], (const char*)g.src + off, len);
+        we = hipMemcpyAsync((char*)g.dst + off, st.buf[w], len, kind, st.ws[w]);
+        if (we == hipSuccess) we = hipStreamSynchronize(st.ws[w]);
+      }
+      if (we != hipSuccess) {
+        err = (int)we;
+        return;
+      }
+    }
+  };
+  const int nw = (int)std::min<size_t>(STAGE_WORKERS, chunks.size());
+  std::vector<std::thread> pool;
+  try {
+    for (int w = 1; w < nw; ++w) pool.emplace_back(worker, w);
+  } catch (...) {  // fewer threads: the started ones and this one take every chunk
+  }
+  worker(0);
+  for (std::thread& t : pool) t.join();
+  return (hipError_t)err.load();
+}
 
 #define API_HIP(expr)                                                             \
   do {                                                                            \
@@ -103,7 +217,7 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
     const double* qsrc = q + b * ndim;
     const double* dqp = qsrc;
     if (!q_on_device) {
-      API_HIP(hipMemcpyAsync(dq.p, qsrc, m * ndim * 8, hipMemcpyHostToDevice, s));
+      API_HIP(staged_copy({{dq.p, qsrc, m * ndim * 8}}, false, s));
       dqp = dq.p;
     }
     // all_finite, src/kernel.rs:67-74 (checked before eps, max_distance, p)
@@ -127,10 +241,8 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
     std::string msg;
     const int rc = kdt::launch_query(t, qpt, dqp, od, oi, s, msg);
     if (rc) return fail(rc, msg);
-    if (!out_on_device) {
-      API_HIP(hipMemcpyAsync(out_d + b * k, dd.p, m * k * 8, hipMemcpyDeviceToHost, s));
-      API_HIP(hipMemcpyAsync(out_i + b * k, di.p, m * k * 8, hipMemcpyDeviceToHost, s));
-    }
+    if (!out_on_device)
+      API_HIP(staged_copy({{out_d + b * k, dd.p, m * k * 8}, {out_i + b * k, di.p, m * k * 8}}, true, s));
   }
   int h = 0;
   API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -164,7 +276,7 @@ int build_impl(const double* data, bool on_device, uint64_t n, uint64_t d, uint6
   double* tmp = nullptr;
   if (!on_device) {
     hipError_t e = hipMalloc(&tmp, n * d * 8);
-    if (e == hipSuccess) e = hipMemcpyAsync(tmp, data, n * d * 8, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = staged_copy({{tmp, data, n * d * 8}}, false, s);
     if (e != hipSuccess) {
       if (tmp) (void)hipFree(tmp);
       delete h;
@@ -221,7 +333,7 @@ int kdt_original_data(const kdt_tree* tree, double* out) {
   DevMem<double> d;
   API_HIP(d.alloc(t.n * t.d, s));
   kdt::launch_original_order(t, d.p, s);
-  API_HIP(hipMemcpyAsync(out, d.p, t.n * t.d * 8, hipMemcpyDeviceToHost, s));
+  API_HIP(staged_copy({{out, d.p, t.n * t.d * 8}}, true, s));
   API_HIP(hipStreamSynchronize(s));
   return KDT_OK;
 }

@@ -150,3 +150,40 @@ def test_float32_and_int_inputs_promote():
     d, i = tree.query(data32[:10], k=1)
     np.testing.assert_array_equal(i[:, 0], np.arange(10))
     np.testing.assert_array_equal(d[:, 0], np.zeros(10))
+
+
+def test_large_host_results_staged():
+    # 2^18 queries x k = 32: 128 MiB of results, copied to the caller's
+    # (pageable) arrays through the library's pinned staging by several host
+    # threads.  Every byte must equal the device-resident path's, and a sample
+    # the oracle's; two calls at once on other threads must too.
+    import torch
+
+    from kdtree import forest as F
+    from oracle import OracleTree
+
+    rng = np.random.default_rng(95)
+    data = rng.uniform(size=(200_000, 3))
+    q = rng.uniform(size=(1 << 18, 3))
+    k = 32
+    tree = KDTree(data, leafsize=32)
+    hd, hi = tree.query(q, k=k)
+    dtree = F.build(data, 32)
+    try:
+        dd, di = F.query_device(dtree, torch.from_numpy(q).cuda(), k)
+        torch.cuda.synchronize()
+        assert hd.tobytes() == dd.cpu().numpy().tobytes()
+        np.testing.assert_array_equal(hi, di.cpu().numpy())
+    finally:
+        dtree.free()
+    od, oi = OracleTree(data, 32).query(q[:3000], k=k, threads=8)
+    np.testing.assert_array_equal(hi[:3000], oi)
+    assert hd[:3000].tobytes() == od.tobytes()
+    out = {}
+    threads = [threading.Thread(target=lambda j=j: out.setdefault(j, tree.query(q, k=k))) for j in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for j in range(2):
+        assert out[j][0].tobytes() == hd.tobytes() and np.array_equal(out[j][1], hi)

@@ -11,7 +11,11 @@ time.  The dispatcher loop:
     arrival order up to the per-launch query cap and dispatches them as ONE
     multi-tree launch (kdt_query_batch_device) on that stream: the requests
     on one tree (with one k) are concatenated into one item, queries go
-    host->device, the launch runs, results go device->host, then an event.
+    host->device, the launch runs, results go device->host, then an event;
+  * the first stream is a lane for small requests (at most `small_max`
+    queries): when it is free it takes every pending small request, so that a
+    small request never waits behind launches of large ones; the other
+    streams take every request in arrival order.
 
 Requests that arrive while earlier launches run are admitted at the next
 dispatch and run concurrently on another stream, so work from different
@@ -55,7 +59,8 @@ class _Launch:
 
 class Server:
     def __init__(self, trees, n_streams: int = 4, hbm_budget_bytes: float = 200e9,
-                 max_queries_per_launch: int = 1 << 18, p: float = 2.0, flags: int = 0):
+                 max_queries_per_launch: int = 1 << 18, p: float = 2.0, flags: int = 0,
+                 small_max: int = 4096):
         import torch
 
         self.torch = torch
@@ -65,6 +70,7 @@ class Server:
         self.cap = max_queries_per_launch
         self.p = p
         self.flags = flags
+        self.small_max = small_max
         self.lib = F._lib()
         self.max_inflight = 0
         self.launches = 0
@@ -140,24 +146,37 @@ class Server:
                     inflight.remove(L)
                     free.append(L.stream)
                     used -= L.nbytes
+            lane = self.streams[0]
+            dispatched = False
             while pending and free:
-                take, nq, nb = [], 0, 0
+                small_lane = lane in free and any(r.queries.shape[0] <= self.small_max for r in pending)
+                if not small_lane and free == [lane]:
+                    break  # only the small-request lane is free, and nothing small waits
+                take, nq, nb, keep = [], 0, 0, deque()
                 while pending and (not take or nq + pending[0].queries.shape[0] <= self.cap):
-                    b = self._bytes(pending[0])
+                    r = pending[0]
+                    if small_lane and r.queries.shape[0] > self.small_max:
+                        keep.append(pending.popleft())  # large: for the other streams
+                        continue
+                    b = self._bytes(r)
                     if used + nb + b > self.budget and (take or inflight):
                         break
-                    r = pending.popleft()
-                    take.append(r)
+                    take.append(pending.popleft())
                     nq += r.queries.shape[0]
                     nb += b
+                keep.extend(pending)
+                pending = keep
                 if not take:
                     break
-                L = self._dispatch(take, free.pop())
+                stream = lane if small_lane else next(x for x in free if x is not lane)
+                free.remove(stream)
+                L = self._dispatch(take, stream)
                 inflight.append(L)
                 used += L.nbytes
                 self.launches += 1
                 self.max_inflight = max(self.max_inflight, len(inflight))
-            if not pending or not free:
+                dispatched = True
+            if not dispatched:
                 time.sleep(0.0002)
         return requests
 

@@ -146,29 +146,12 @@ class Server:
                     inflight.remove(L)
                     free.append(L.stream)
                     used -= L.nbytes
-            lane = self.streams[0]
             dispatched = False
             while pending and free:
-                small_lane = lane in free and any(r.queries.shape[0] <= self.small_max for r in pending)
-                if not small_lane and free == [lane]:
-                    break  # only the small-request lane is free, and nothing small waits
-                take, nq, nb, keep = [], 0, 0, deque()
-                while pending and (not take or nq + pending[0].queries.shape[0] <= self.cap):
-                    r = pending[0]
-                    if small_lane and r.queries.shape[0] > self.small_max:
-                        keep.append(pending.popleft())  # large: for the other streams
-                        continue
-                    b = self._bytes(r)
-                    if used + nb + b > self.budget and (take or inflight):
-                        break
-                    take.append(pending.popleft())
-                    nq += r.queries.shape[0]
-                    nb += b
-                keep.extend(pending)
-                pending = keep
+                take, stream, pending = plan_launch(pending, free, self.streams[0], self.small_max, self.cap,
+                                                    self.budget - used, bool(inflight), self._bytes)
                 if not take:
                     break
-                stream = lane if small_lane else next(x for x in free if x is not lane)
                 free.remove(stream)
                 L = self._dispatch(take, stream)
                 inflight.append(L)
@@ -179,6 +162,38 @@ class Server:
             if not dispatched:
                 time.sleep(0.0002)
         return requests
+
+
+def plan_launch(pending, free, lane, small_max, cap, budget_left, busy, nbytes):
+    """The next launch: (requests, stream, requests still pending).
+
+    `pending` holds requests in arrival order, `free` the idle streams, `lane`
+    the stream kept for small requests (at most `small_max` queries).  When the
+    lane is idle and a small request waits, the launch takes pending small
+    requests only, on the lane; otherwise it takes requests in arrival order
+    on another idle stream.  Either way it stops at `cap` queries (the first
+    request always fits) and at `budget_left` bytes of `nbytes(request)`,
+    which a launch may exceed only when nothing else runs (`busy` false).
+    Returns no requests when nothing can start."""
+    small_lane = lane in free and any(r.queries.shape[0] <= small_max for r in pending)
+    others = [x for x in free if x is not lane]
+    if not small_lane and not others:
+        return [], None, pending
+    pending = deque(pending)
+    take, nq, nb, keep = [], 0, 0, deque()
+    while pending and (not take or nq + pending[0].queries.shape[0] <= cap):
+        r = pending[0]
+        if small_lane and r.queries.shape[0] > small_max:
+            keep.append(pending.popleft())  # large: for the other streams
+            continue
+        b = nbytes(r)
+        if nb + b > budget_left and (take or busy):
+            break
+        take.append(pending.popleft())
+        nq += r.queries.shape[0]
+        nb += b
+    keep.extend(pending)
+    return take, (lane if small_lane else others[0]) if take else None, keep
 
 
 def latency_stats(requests):

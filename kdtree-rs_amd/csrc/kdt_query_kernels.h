@@ -666,7 +666,14 @@ __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s
   const uint64_t ngroups = (uint64_t)gridDim.x * gpb;
   typename std::conditional<LDSLIST, LdsList, RegList<G>>::type best;
   if constexpr (LDSLIST) { best.ld = l_d; best.li = l_i; }
-  uint64_t qs = (uint64_t)blockIdx.x * gpb + grp;
+  // Workgroups are handed to the 8 XCDs in turn (workgroup b to XCD b mod 8),
+  // each XCD with its own L2: XCD x gets a contiguous eighth of the blocks, so
+  // that neighbouring queries (large batches run in leaf order) share one L2
+  // (C3 117 -> 103 ms, C2 -2%).  A bijection on the blocks below the largest
+  // multiple of 8; the rest keep their index.
+  const uint32_t full = gridDim.x & ~7u;
+  const uint32_t blk = blockIdx.x < full ? (blockIdx.x & 7u) * (full >> 3) + (blockIdx.x >> 3) : blockIdx.x;
+  uint64_t qs = (uint64_t)blk * gpb + grp;
   bool first = true;
   auto next = [&](uint64_t& qi) -> bool {
     for (;;) {

@@ -86,7 +86,9 @@ struct SArgs {
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
   uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
   uint64_t nq;
-  uint32_t D, n, tiles_per_slice, n_tiles, slices;
+  uint32_t D, n, n_tiles, slices;
+  uint32_t tiles_per_slice;  // contiguous slices (k <= KP)
+  uint32_t interleave;       // k > KP: slice y takes tiles y, y + slices, ...
 };
 
 // Waves (32-query tiles) per workgroup sharing each point tile: 8 measured
@@ -295,8 +297,27 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
   sc.cnt = 0;
   sc.buf = s_buf + tid;
 
-  const uint32_t t0 = blockIdx.y * a.tiles_per_slice;
-  const uint32_t t1 = min(a.n_tiles, t0 + a.tiles_per_slice);
+  // Local tile i of slice y is global point tile tb + i ts.  The rows are in
+  // tree order, so a contiguous range of tiles is one region of space and holds
+  // most of a query's near neighbours: fine while its two lane lists of KP hold
+  // them (k <= KP: contiguous, tb = y * tiles_per_slice, ts = 1), not beyond
+  // (k > KP: every S-th tile, tb = y, ts = S, so that each slice samples all of
+  // space; 16-d uniform data, k = 32: 12-28% of the queries overflowed a list
+  // and took the traversal with contiguous slices, none interleaved).  The
+  // loop below runs over local indices.
+  const uint32_t S = a.slices, sy = blockIdx.y;
+  uint32_t tb, ts, t1;
+  if (a.interleave) {
+    tb = sy;
+    ts = S;
+    t1 = sy < a.n_tiles ? (a.n_tiles - sy + S - 1) / S : 0;
+  } else {
+    tb = sy * a.tiles_per_slice;
+    ts = 1;
+    t1 = tb < a.n_tiles ? min(a.tiles_per_slice, a.n_tiles - tb) : 0;
+  }
+  const uint32_t t0 = 0;
+#define KDT_TILE(i) (tb + (i) * ts)
   // Tile t sits in LDS stage (t - t0) & 1 while it is contracted; tile t + 1
   // is written to the other stage in the same iteration (its readers, tile
   // t - 1, finished before the barrier that ended the previous iteration) and
@@ -310,18 +331,18 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
 #define KDT_SCREEN_STEP(T, B, N0, N1, NP)                                                 \
   do {                                                                                    \
     if ((T) + 1 < t1) stage_store<KS>(N0, N1, NP, tid, s_rows[(B) ^ 1], s_pn[(B) ^ 1]); \
-    stage_load<KS>(a, min((T) + 3, t1 - 1), tid, N0, N1, NP);                           \
-    if (active && (T) < t1) sc.tile((T), j, h, s_rows[B], s_pn[B]);                       \
+    stage_load<KS>(a, KDT_TILE(min((T) + 3, t1 - 1)), tid, N0, N1, NP);                 \
+    if (active && (T) < t1) sc.tile(KDT_TILE(T), j, h, s_rows[B], s_pn[B]);               \
     __syncthreads();                                                                      \
   } while (0)
   if (t0 < t1) {  // uniform over the workgroup
     uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
     uint32_t xp = 0, yp = 0;
-    stage_load<KS>(a, t0, tid, x0, x1, xp);
+    stage_load<KS>(a, KDT_TILE(t0), tid, x0, x1, xp);
     stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
-    stage_load<KS>(a, min(t0 + 1, t1 - 1), tid, x0, x1, xp);
+    stage_load<KS>(a, KDT_TILE(min(t0 + 1, t1 - 1)), tid, x0, x1, xp);
     __builtin_amdgcn_sched_barrier(0);  // keep the issue order of the loop: x's loads, then y's
-    stage_load<KS>(a, min(t0 + 2, t1 - 1), tid, y0, y1, yp);
+    stage_load<KS>(a, KDT_TILE(min(t0 + 2, t1 - 1)), tid, y0, y1, yp);
     __syncthreads();
     for (uint32_t t = t0; t < t1; t += 2) {
       KDT_SCREEN_STEP(t, 0, x0, x1, xp);
@@ -329,6 +350,7 @@ __global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
     }
   }
 #undef KDT_SCREEN_STEP
+#undef KDT_TILE
   if (active) sc.merge();  // wave-uniform
   if (qvalid) {
     const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
@@ -654,7 +676,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
   // Slices of the points per query tile: enough waves to fill the device,
   // at least 8 tiles per slice.
-  // For k > KP, at least k / 4 slices: a query's top k then spread over at
+  // For k > KP, at least k / 4 interleaved slices: a query's top k then spread over at
   // least k / 2 lists of KP, about two per list, so a list rarely fills below
   // tau_A (the completeness check flags the query when one does).
   uint64_t slices = std::max<uint64_t>(1, (4096 + qtiles - 1) / qtiles);
@@ -690,8 +712,9 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   sa.nq = nq;
   sa.D = (uint32_t)t.d;
   sa.n = (uint32_t)t.n;
-  sa.tiles_per_slice = per;
   sa.n_tiles = n_tiles;
+  sa.tiles_per_slice = per;
+  sa.interleave = k > (uint32_t)KP;
   sa.slices = (uint32_t)slices;
   const dim3 grid((uint32_t)((qtiles + SW - 1) / SW), (uint32_t)slices);
   const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;

@@ -129,6 +129,28 @@ def test_matrix_screen_narrow_rows(case):
     np.testing.assert_array_equal(ti, oi)
 
 
+def test_k32_uniform_rows_stay_on_the_screen():
+    # Rows in tree order put a query's near neighbours in one stretch of point
+    # tiles.  For k > 16 the point slices interleave their tiles; as contiguous
+    # ranges, the two lane lists of the slice holding that stretch overflowed
+    # and 12% of these queries were left to the (much slower) traversal.
+    def f32_uniform(seed, n):
+        return np.random.default_rng(seed).uniform(size=(n, 16)).astype(np.float32).astype(np.float64)
+
+    data, q = f32_uniform(37, 100_000), f32_uniform(38, 20_000)
+    tree = KDTree(data, leafsize=32)
+    pd, pi = tree.query(q, k=32)  # the FP64 traversal
+    sd, si = screen_only(tree, q, 32, None, MATRIX_SCREEN)
+    done = si[:, 0] != -7
+    print(f"screen answered {done.mean():.4f} of the queries", flush=True)
+    assert done.mean() >= 0.99
+    np.testing.assert_array_equal(si[done], pi[done])
+    assert sd[done].tobytes() == pd[done].tobytes()
+    od, oi = OracleTree(data, 32).query(q[:300], k=32, threads=8)
+    np.testing.assert_array_equal(pi[:300], oi)
+    assert pd[:300].tobytes() == od.tobytes()
+
+
 @pytest.mark.parametrize("k", [10, 32])
 def test_screen_large_batch_device(k):
     """2^14 queries on the device: few point slices per query tile (k = 32

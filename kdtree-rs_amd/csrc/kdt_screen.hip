@@ -91,21 +91,26 @@ struct SArgs {
   uint32_t interleave;       // k > KP: slice y takes tiles y, y + slices, ...
 };
 
-// Waves (32-query tiles) per workgroup sharing each point tile: 8 measured
-// fastest (4: each point tile read twice as often from L2; 16: fewer
-// workgroups than CUs can hold, a partial second round).
-constexpr int SW = 8;
+// Waves (32-query tiles) per workgroup sharing each point tile.  Rows of 64
+// columns: 8 measured fastest (4: each point tile read twice as often from L2,
+// 33.0 vs 22.8 ms; 16: fewer workgroups than CUs can hold, a partial second
+// round).  Rows of 16 columns: 4 (the per-tile barrier waits on fewer waves;
+// 16-d uniform rows 1.8 vs 2.3 ms, normal rows 3% faster).
+template <int KS>
+constexpr int screen_waves() { return KS <= 8 ? 4 : 8; }
 
 // Point tile t, 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS], KS / 2 uint4),
-// staged global -> registers -> LDS by the whole workgroup: thread u copies
-// uint4 u, u + 64 SW, ... of the tile (coalesced), and threads 0..31 its split |p~|^2.
+// staged global -> registers -> LDS by the whole workgroup of SW = screen_waves
+// waves: thread u copies uint4 u, u + 64 SW, ... of the tile (coalesced), and
+// threads 0..31 its split |p~|^2.
 // (Nothing is read past the tile: TU and 32 bound every index.)
 template <int KS>
 struct Stage {
   static constexpr int RU = KS / 2;                          // uint4 per row
   static constexpr int TU = 32 * RU;                         // uint4 per tile
+  static constexpr int SW = screen_waves<KS>();
   static constexpr int PT = (TU + 64 * SW - 1) / (64 * SW);  // uint4 per thread
-  static_assert(PT <= 2, "two staging registers per thread (SW >= 4)");
+  static_assert(PT <= 2, "two staging registers per thread");
 };
 
 // The stage is three plain registers (v0, v1, pn), not an aggregate: an
@@ -116,7 +121,7 @@ __device__ __forceinline__ void stage_load(const SArgs& a, uint32_t t, uint32_t 
                                            uint32_t& pn) {
   const uint4* src = reinterpret_cast<const uint4*>(a.rows_bf) + (size_t)t * Stage<KS>::TU;
   v0 = src[min(tid, (uint32_t)Stage<KS>::TU - 1)];
-  if (Stage<KS>::PT > 1) v1 = src[min(tid + 64 * SW, (uint32_t)Stage<KS>::TU - 1)];
+  if (Stage<KS>::PT > 1) v1 = src[min(tid + 64 * Stage<KS>::SW, (uint32_t)Stage<KS>::TU - 1)];
   pn = a.pnorm_bf[(size_t)t * 32 + (tid & 31)];
 }
 
@@ -127,7 +132,7 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, ui
                                             uint4* rows, uint32_t* pn) {
   constexpr int RU = Stage<KS>::RU;
   if (tid < (uint32_t)Stage<KS>::TU) rows[(tid / RU) * (RU + 1) + tid % RU] = v0;
-  const uint32_t u = tid + 64 * SW;
+  const uint32_t u = tid + 64 * Stage<KS>::SW;
   if (Stage<KS>::PT > 1 && u < (uint32_t)Stage<KS>::TU) rows[(u / RU) * (RU + 1) + u % RU] = v1;
   if (tid < 32) pn[tid] = pnv;
 }
@@ -171,6 +176,7 @@ struct Screen {
   float thr;        // la[KP - 1] at the last merge
   uint32_t cnt;     // entries in this lane's buffer
   uint2* buf;       // this lane's buffer: slot e at buf[e * 64 * SW], (A bits, position)
+  static constexpr int SW = screen_waves<KS>();
 
   __device__ __forceinline__ void insert(float A, uint32_t row) {  // dropping the largest
 #pragma unroll
@@ -254,7 +260,8 @@ struct Screen {
 };
 
 template <int KS>
-__global__ __launch_bounds__(64 * SW) void k_screen(SArgs a) {
+__global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
+  constexpr int SW = screen_waves<KS>();
   constexpr int RS = KS / 2 + 1;
   __shared__ uint4 s_rows[2][32 * RS];
   __shared__ __align__(16) uint32_t s_pn[2][32];
@@ -716,11 +723,12 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   sa.tiles_per_slice = per;
   sa.interleave = k > (uint32_t)KP;
   sa.slices = (uint32_t)slices;
-  const dim3 grid((uint32_t)((qtiles + SW - 1) / SW), (uint32_t)slices);
   const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;
-  if (ks == 8) k_screen<8><<<grid, 64 * SW, 0, s>>>(sa);
-  else if (ks == 16) k_screen<16><<<grid, 64 * SW, 0, s>>>(sa);
-  else k_screen<32><<<grid, 64 * SW, 0, s>>>(sa);
+  const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
+  const dim3 grid((uint32_t)((qtiles + sw - 1) / sw), (uint32_t)slices);
+  if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(sa);
+  else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(sa);
+  else k_screen<32><<<grid, 64 * sw, 0, s>>>(sa);
 
   SelArgs se{};
   se.part_a = part_a;

@@ -94,10 +94,11 @@ struct SArgs {
 // Waves (32-query tiles) per workgroup sharing each point tile.  Rows of 64
 // columns: 8 measured fastest (4: each point tile read twice as often from L2,
 // 33.0 vs 22.8 ms; 16: fewer workgroups than CUs can hold, a partial second
-// round).  Rows of 16 columns: 4 (the per-tile barrier waits on fewer waves;
-// 16-d uniform rows 1.8 vs 2.3 ms, normal rows 3% faster).
+// round).  Rows of 16 and 32 columns: 4 (the per-tile barrier waits on fewer
+// waves; 16-d uniform rows 1.8 vs 2.3 ms, 16-d normal rows 3% faster; 24-d
+// uniform rows at k = 32 5.1 vs 6.2 ms, 32-d normal rows at k = 10 3% slower).
 template <int KS>
-constexpr int screen_waves() { return KS <= 8 ? 4 : 8; }
+constexpr int screen_waves() { return KS <= 16 ? 4 : 8; }
 
 // Point tile t, 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS], KS / 2 uint4),
 // staged global -> registers -> LDS by the whole workgroup of SW = screen_waves

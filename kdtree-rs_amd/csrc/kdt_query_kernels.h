@@ -296,6 +296,46 @@ __device__ __forceinline__ float screen_threshold(double bound, double E, const 
   return __double2float_ru(t);
 }
 
+// ------------------------------------------------------------------ shuffles
+// Within a group of 16 lanes (one DPP row) some shuffles are DPP moves, which
+// stay in the VALU instead of a round trip through the LDS crossbar: the shift
+// up by one (row_shr:1, lane 0 keeps its value, as __shfl_up does) and the xor
+// exchanges with stride 1, 2 (quad_perm) and 8 (row_ror:8).  Other widths and
+// strides use __shfl_up / __shfl_xor.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const uint64_t lo = dpp32<CTRL>((uint32_t)b), hi = dpp32<CTRL>((uint32_t)(b >> 32));
+  return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+template <int G, typename T>
+__device__ __forceinline__ T shfl_up1(T x) {
+  if constexpr (G == 16) {
+    if constexpr (sizeof(T) == 8) return dpp64<0x111>(x);
+    else return (T)dpp32<0x111>((uint32_t)x);
+  } else {
+    return __shfl_up(x, 1, G);
+  }
+}
+template <int G, typename T>
+__device__ __forceinline__ T shfl_xor_g(T x, uint32_t stride) {
+  if constexpr (G == 16) {
+    if (stride == 1 || stride == 2 || stride == 8) {
+      if constexpr (sizeof(T) == 8) {
+        return stride == 1 ? dpp64<0xB1>(x) : stride == 2 ? dpp64<0x4E>(x) : dpp64<0x128>(x);
+      } else {
+        const uint32_t u = (uint32_t)x;
+        return (T)(stride == 1 ? dpp32<0xB1>(u) : stride == 2 ? dpp32<0x4E>(u) : dpp32<0x128>(u));
+      }
+    }
+  }
+  return __shfl_xor(x, stride, G);
+}
+
 // ------------------------------------------------------------------ k lists
 // One entry per lane (k <= G).  Lane j holds the j-th best.
 template <int G>
@@ -310,8 +350,8 @@ struct RegList {
     const bool less = lane < cnt && (kd < d || (kd == d && ki < x));
     const uint32_t pos = (uint32_t)__popcll((__ballot(less) >> gshift) & gmask);
     if (pos < a.k) {
-      const double ud = __shfl_up(kd, 1, G);
-      const uint32_t ui = __shfl_up(ki, 1, G);
+      const double ud = shfl_up1<G>(kd);
+      const uint32_t ui = shfl_up1<G>(ki);
       if (lane == pos) { kd = d; ki = x; }
       else if (lane > pos && lane < a.k) { kd = ud; ki = ui; }
       if (cnt < a.k) ++cnt;
@@ -336,8 +376,8 @@ struct RegList {
     for (uint32_t size = 2; size <= G; size <<= 1) {
 #pragma unroll
       for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-        const double pd = __shfl_xor(bd, stride, G);
-        const uint32_t pi = __shfl_xor(bi, stride, G);
+        const double pd = shfl_xor_g<G>(bd, stride);
+        const uint32_t pi = shfl_xor_g<G>(bi, stride);
         const bool up = (lane & size) == 0 || size == G;
         const bool lower = (lane & stride) == 0;
         const bool take = (lower == up) ? lless(pd, pi, bd, bi) : lless(bd, bi, pd, pi);
@@ -350,8 +390,8 @@ struct RegList {
     if (lless(rd, ri, kd, ki)) { kd = rd; ki = ri; }
 #pragma unroll
     for (uint32_t stride = G >> 1; stride > 0; stride >>= 1) {
-      const double pd = __shfl_xor(kd, stride, G);
-      const uint32_t pi = __shfl_xor(ki, stride, G);
+      const double pd = shfl_xor_g<G>(kd, stride);
+      const uint32_t pi = shfl_xor_g<G>(ki, stride);
       const bool lower = (lane & stride) == 0;
       const bool take = lower ? lless(pd, pi, kd, ki) : lless(kd, ki, pd, pi);
       if (take) { kd = pd; ki = pi; }

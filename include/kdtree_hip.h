@@ -64,9 +64,10 @@ enum kdt_error {
                                         (hipMallocAsync) for the leaf-order sort. */
 
 #define KDT_QUERY_MATRIX_SCREEN 0x8u /* with KDT_QUERY_FP32_SCREEN: the matrix-core
-                                        screen also for 16 <= ndim < 32, where the
-                                        traversal still prunes part of the tree (a
-                                        per-tree choice, made by timing both) */
+                                        screen also for 8 <= ndim < 32 (rows padded
+                                        to 16 or 32 columns), where the traversal
+                                        still prunes part of the tree (a per-tree
+                                        choice, made by timing both) */
 #define KDT_QUERY_GROUP_SHIFT 4u /* bits 4-6: lanes per query in the traversal, 0 = chosen
                                     from k (16 for k <= 16, 32 for k <= 32, else 64),
                                     1 = 8, 2 = 16, 3 = 32, 4 = 64 lanes (ignored when fewer

@@ -838,7 +838,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   KDT_HIP_CHECK(boxes.alloc(t.n_nodes * 2 * D));
   KDT_HIP_CHECK(pts.alloc(n * D));
   if (D <= 16) KDT_HIP_CHECK(pts32.alloc(n * D));  // the FP32 screen of the compiled widths
-  const bool wide = D >= 16 && D <= 64;
+  const bool wide = D >= 8 && D <= 64;  // rows for the matrix-core screen
   const uint32_t stride32 = D <= 16 ? 16 : D <= 32 ? 32 : 64;
   const uint64_t n_pad = (n + 31) / 32 * 32;
   if (wide) {

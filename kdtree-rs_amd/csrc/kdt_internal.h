@@ -9,7 +9,7 @@
 //   idx    u32[n_points]               original row of tree position
 //   rows   f64[n_points][ndim]         rows in tree order (row-major), the
 //                                      reference's Tree::data
-//   rows_bf bf16[n_pad][2][stride]     16 <= ndim <= 64 only: rows in tree order
+//   rows_bf bf16[n_pad][2][stride]     8 <= ndim <= 64 only: rows in tree order
 //                                      rounded to f32 (v), zero-padded to stride
 //                                      16, 32 or 64 and to n_pad = 32 * ceil(n / 32)
 //                                      rows, as bf16 hi = bf16(v) then lo =
@@ -66,7 +66,7 @@ struct QueryParams {
   double p = 2.0, cutoff = 0.0, eps_factor = 1.0;
   bool fp32_screen = false;
   bool screen_only = false;    // diagnostic: no traversal after the matrix-core screen
-  bool matrix_screen = false;  // matrix-core screen also for 16 <= ndim < 32
+  bool matrix_screen = false;  // matrix-core screen also for 8 <= ndim < 32
   int group = 0;               // lanes per query requested (8, 16, 32, 64), 0 = from k
   bool allow_order = true;    // large batches may run in leaf order
   bool allow_scratch = true;  // stream-ordered scratch allowed (the matrix-core screen needs it);
@@ -105,7 +105,7 @@ void set_last_error(std::string msg);
 int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
                  int64_t* d_idx, hipStream_t s, std::string& msg);
 
-// Matrix-core screen for 16 <= ndim <= 64 (kdt_screen.hip).
+// Matrix-core screen for 8 <= ndim <= 64 (kdt_screen.hip).
 struct KArgs;
 bool screen_applies(const Tree& t, const QueryParams& qp);
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg);

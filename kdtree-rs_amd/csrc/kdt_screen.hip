@@ -1,5 +1,5 @@
 // kdt_screen.hip -- exact k-NN for wide rows (32 <= ndim <= 64, and
-// 16 <= ndim < 32 on request; p = 2, eps = 0) by an MFMA distance screen, an
+// 8 <= ndim < 32 on request; p = 2, eps = 0) by an MFMA distance screen, an
 // FP64 recheck and a visit certificate.
 //
 // Replaces, for these widths, the leaf scans of the reference's descent
@@ -512,11 +512,31 @@ struct CheckArgs {
   int64_t* __restrict__ out_i;
   uint64_t nq;
   uint32_t D, k, n;
+  uint32_t leafsize;
   double cutoff;
 };
 
 __device__ __forceinline__ bool key_less(double da, uint32_t ia, double db, uint32_t ib) {
   return da < db || (da == db && ia < ib);
+}
+
+// Whether tree position p lies in a complete chunk of P rows counted from the
+// first row of its leaf: the Packed recipes of 4- and 8-column rows use the
+// pair / chunk order only there (src/kernel.rs:447).  The tree shape depends on
+// (n, leafsize) only: a node of more than leafsize rows keeps its first len / 2
+// rows on the left (src/build.rs:166-173), so the leaf follows from p alone.
+__device__ __forceinline__ bool in_full_chunk(uint32_t p, uint32_t n, uint32_t leafsize, uint32_t P) {
+  uint32_t start = 0, len = n;
+  while (len > leafsize) {
+    const uint32_t mid = len / 2;
+    if (p < start + mid) {
+      len = mid;
+    } else {
+      start += mid;
+      len -= mid;
+    }
+  }
+  return p - start < len / P * P;
 }
 
 // One wave per query: FP64 recheck of the candidates, top-k, certificate.
@@ -548,7 +568,9 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
     return;
   }
   if (lane < c) {
-    const double v = point_dist_d<M_L2, 0>(q, a.rows + (size_t)pos * D, 1, D, true, 2.0);
+    const bool full = D == 8 ? in_full_chunk(pos, a.n, a.leafsize, 8) : D == 4 ? in_full_chunk(pos, a.n, a.leafsize, 2)
+                                                                        : true;
+    const double v = point_dist_d<M_L2, 0>(q, a.rows + (size_t)pos * D, 1, D, full, 2.0);
     if (v <= a.cutoff) {
       d = v;
       id = a.idx[pos];
@@ -674,7 +696,7 @@ int dev_alloc(T** p, size_t count, hipStream_t s) {
 bool screen_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.allow_scratch && qp.metric == M_L2 && qp.eps_factor == 1.0 &&
          qp.k <= (uint64_t)KMAX && t.rows_bf &&
-         qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 16)) && t.d <= 64;
+         qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 8)) && t.d <= 64;
 }
 
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg) {
@@ -762,6 +784,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   ca.D = (uint32_t)t.d;
   ca.k = k;
   ca.n = (uint32_t)t.n;
+  ca.leafsize = (uint32_t)std::min<uint64_t>(t.leafsize, 0xFFFFFFFFu);
   ca.cutoff = qp.cutoff;
   k_screen_recheck<<<(uint32_t)nq, 64, (size_t)k * t.d * 8, s>>>(ca);
 

@@ -105,12 +105,21 @@ NARROW = {
     "normal-4000x24-k3": (lambda: np.random.default_rng(27).normal(size=(4000, 24)),
                           lambda: np.random.default_rng(28).normal(size=(129, 24)), 3, 32),
     "normal-f32-12000x16-k32": (lambda: f32_normal(35, 12_000, 16), lambda: f32_normal(36, 500, 16), 32, 32),
+    # 8 <= ndim < 16: rows padded to 16 columns
+    "uniform-f32-20000x8-k8": (lambda: np.random.default_rng(39).uniform(size=(20_000, 8)).astype(np.float32)
+                               .astype(np.float64),
+                               lambda: np.random.default_rng(40).uniform(size=(600, 8)).astype(np.float32)
+                               .astype(np.float64), 8, 32),
+    "normal-9000x12-k32-ls16": (lambda: np.random.default_rng(41).normal(size=(9000, 12)),
+                                lambda: np.random.default_rng(42).normal(size=(400, 12)), 32, 16),
+    "normal-5000x8-k1": (lambda: np.random.default_rng(43).normal(size=(5000, 8)),
+                         lambda: np.random.default_rng(44).normal(size=(257, 8)), 1, 32),
 }
 
 
 @pytest.mark.parametrize("case", NARROW.values(), ids=NARROW.keys())
 def test_matrix_screen_narrow_rows(case):
-    """16 <= ndim < 32 takes the matrix-core screen only when asked."""
+    """8 <= ndim < 32 takes the matrix-core screen only when asked."""
     make_data, make_q, k, leafsize = case
     data = np.ascontiguousarray(make_data())
     q = np.ascontiguousarray(make_q())

@@ -130,10 +130,11 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
     group width (lanes per query that hold k), keep the fastest pair; then,
     where the matrix-core screen applies (16 <= d <= 64, k <= 32, for queries
     with p = 2 and eps = 0), time it too and keep it in `flags` if faster.
-    (The screen also takes 8 <= d < 16 when asked; it is not offered here: on a
-    sample below the traversal's leaf-ordering size the traversal looks slower
-    than it is on a full item, and on the batching workload choosing the screen
-    for 8-d trees from such samples lost 20%.)
+    (The screen also takes 8 <= d < 16 when asked; it is not offered here: on the
+    batching workload choosing it for 8-d trees lost 13-20% whether the sample
+    was timed as given or repeated to the traversal's leaf-ordering size, since
+    each such item then leaves the shared launch for a launch sequence of its
+    own.)
 
     The leafsize changes the tree, and for 4- and 8-dimensional rows the
     reference's summation order inside a leaf, so the chosen leafsize is part

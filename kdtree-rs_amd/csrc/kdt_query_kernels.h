@@ -299,12 +299,15 @@ __device__ __forceinline__ float screen_threshold(double bound, double E, const 
 // ------------------------------------------------------------------ shuffles
 // Within a group of 16 lanes (one DPP row) some shuffles are DPP moves, which
 // stay in the VALU instead of a round trip through the LDS crossbar: the shift
-// up by one (row_shr:1, lane 0 keeps its value, as __shfl_up does) and the xor
-// exchanges with stride 1, 2 (quad_perm) and 8 (row_ror:8).  Other widths and
-// strides use __shfl_up / __shfl_xor.
+// up by one (row_shr:1) and the xor exchanges with stride 1, 2 (quad_perm) and
+// 8 (row_ror:8).  Other widths and strides use __shfl_up / __shfl_xor.  The
+// moves carry no old value (bound_ctrl: a lane without a source reads 0), so
+// each is one v_mov_b32_dpp with no copy in front; the only such lane is lane 0
+// of a row under row_shr:1, and its shifted value is never used (a k list's
+// lane 0 never takes the entry above it).
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp32(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, CTRL, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp64(double x) {
@@ -365,8 +368,17 @@ struct RegList {
   // lower half of (list, reversed offers) is kept (half-cleaner) and merged
   // into order.  Returns the new bound.
   static constexpr bool kBulk = true;
+  // Compare-exchange steps: a lane keeps the smaller of (own, partner) when
+  // `lower == up`, else the larger.  "Partner strictly before own" (one
+  // lexicographic test, no branch) decides both: the larger is kept when the
+  // partner is not before own, which for two equal pairs (only padding
+  // entries can be equal: indices are distinct, distances never NaN) swaps
+  // identical values.  So take = before(partner, own) XOR (lower != up).
   __device__ __forceinline__ double insert_many(double d, uint32_t x, bool valid, uint32_t nvalid, double bound,
                                                 uint32_t lane, const KArgs& a) {
+    auto before = [](double d1, uint32_t i1, double d2, uint32_t i2) {
+      return (d1 < d2) | ((d1 == d2) & (i1 < i2));
+    };
     auto lless = [](double d1, uint32_t i1, double d2, uint32_t i2) {
       return d1 < d2 || (d1 == d2 && i1 < i2);
     };
@@ -380,8 +392,9 @@ struct RegList {
         const uint32_t pi = shfl_xor_g<G>(bi, stride);
         const bool up = (lane & size) == 0 || size == G;
         const bool lower = (lane & stride) == 0;
-        const bool take = (lower == up) ? lless(pd, pi, bd, bi) : lless(bd, bi, pd, pi);
-        if (take) { bd = pd; bi = pi; }
+        const bool take = before(pd, pi, bd, bi) != (lower != up);
+        bd = take ? pd : bd;
+        bi = take ? pi : bi;
       }
     }
     // lane j: min(list[j], offers[G-1-j]) -- the G smallest of both, bitonic
@@ -393,8 +406,9 @@ struct RegList {
       const double pd = shfl_xor_g<G>(kd, stride);
       const uint32_t pi = shfl_xor_g<G>(ki, stride);
       const bool lower = (lane & stride) == 0;
-      const bool take = lower ? lless(pd, pi, kd, ki) : lless(kd, ki, pd, pi);
-      if (take) { kd = pd; ki = pi; }
+      const bool take = before(pd, pi, kd, ki) != !lower;
+      kd = take ? pd : kd;
+      ki = take ? pi : ki;
     }
     cnt = cnt + nvalid < a.k ? cnt + nvalid : a.k;
     if (lane >= cnt) { kd = INFINITY; ki = 0xFFFFFFFFu; }

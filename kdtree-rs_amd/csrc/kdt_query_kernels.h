@@ -350,13 +350,14 @@ struct RegList {
   // BestK::consider, src/query.rs:462-481.  Returns the new bound.
   __device__ __forceinline__ double insert(double d, uint32_t x, double bound, uint32_t lane,
                                            uint32_t gshift, uint64_t gmask, const KArgs& a) {
-    const bool less = lane < cnt && (kd < d || (kd == d && ki < x));
+    const bool less = (lane < cnt) & ((kd < d) | ((kd == d) & (ki < x)));
     const uint32_t pos = (uint32_t)__popcll((__ballot(less) >> gshift) & gmask);
     if (pos < a.k) {
       const double ud = shfl_up1<G>(kd);
       const uint32_t ui = shfl_up1<G>(ki);
-      if (lane == pos) { kd = d; ki = x; }
-      else if (lane > pos && lane < a.k) { kd = ud; ki = ui; }
+      const bool at = lane == pos, up = (lane > pos) & (lane < a.k);
+      kd = at ? d : up ? ud : kd;
+      ki = at ? x : up ? ui : ki;
       if (cnt < a.k) ++cnt;
       if (cnt == a.k) bound = fmin(__shfl(kd, a.k - 1, G), a.cutoff);  // BestK::bound, :453-459
     }
@@ -622,26 +623,32 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
         dist = point_dist_d<M, DT>(q, base + o, L, D, o < full, a.p);
         return dist <= bound;
       };
+      // The group's offers go in lane order; after each insert an offer stays
+      // pending while it is not yet taken and still within the new bound.
+      // `gm` is the group's window of the wave's ballot of pending offers.
+      using GM = typename std::conditional<G == 64, uint64_t, uint32_t>::type;
       auto offer = [&](bool hit, double dist, uint32_t o) {
-        uint64_t m = (__ballot(hit && dist <= bound) >> gshift) & gmask;
-        if (!m) return;
+        bool pending = hit && dist <= bound;
+        GM gm = (GM)((__ballot(pending) >> gshift) & gmask);
+        if (!gm) return;
         const uint32_t myidx = hit ? a.idx[start + o] : 0u;
         if constexpr (List::kBulk) {
-          const uint32_t nv = (uint32_t)__popcll(m);
+          const uint32_t nv = (uint32_t)__popcll((uint64_t)gm);
           if (nv >= BULK_MIN) {
-            bound = best.insert_many(dist, myidx, hit && dist <= bound, nv, bound, lane, a);
+            bound = best.insert_many(dist, myidx, pending, nv, bound, lane, a);
             if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
             return;
           }
         }
         do {
-          const int f = __ffsll((unsigned long long)m) - 1;
-          const double df = __shfl(dist, f, G);
-          const uint32_t xf = __shfl(myidx, f, G);
+          const uint32_t f = G == 64 ? (uint32_t)__ffsll((unsigned long long)gm) - 1u
+                                     : (uint32_t)__ffs((uint32_t)gm) - 1u;
+          const double df = __shfl(dist, (int)f, G);
+          const uint32_t xf = __shfl(myidx, (int)f, G);
           bound = best.insert(df, xf, bound, lane, gshift, gmask, a);
-          m &= m - 1;
-          m &= (__ballot(hit && dist <= bound) >> gshift) & gmask;
-        } while (m);
+          pending = pending & (lane != f) & (dist <= bound);
+          gm = (GM)((__ballot(pending) >> gshift) & gmask);
+        } while (gm);
         if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
       };
       for (uint32_t c0 = 0; c0 < L; c0 += 2 * G) {

@@ -66,8 +66,11 @@ constexpr int STK_MAX = 64;
 template <int DT>
 constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : DT == 8 ? 6 : 1; }
 // Hits in one leaf chunk from which a group inserts them with one bitonic
-// merge instead of one by one (the merge costs about ten single inserts).
-constexpr uint32_t BULK_MIN = 10;
+// merge instead of one by one: for groups of up to 16 lanes (all exchanges DPP
+// moves, 14 steps) the merge costs about five single inserts; wider groups
+// take more steps, some through the LDS crossbar, and keep 10 (swept 4-16).
+template <int G>
+constexpr uint32_t bulk_min() { return G <= 16 ? 5u : 10u; }
 constexpr uint32_t KIND_FAR = 0, KIND_BOXFAR = 1, KIND_BOXNEAR = 2, KIND_RESTORE = 3;
 
 struct KArgs {
@@ -297,10 +300,11 @@ __device__ __forceinline__ float screen_threshold(double bound, double E, const 
 }
 
 // ------------------------------------------------------------------ shuffles
-// Within a group of 16 lanes (one DPP row) some shuffles are DPP moves, which
-// stay in the VALU instead of a round trip through the LDS crossbar: the shift
-// up by one (row_shr:1) and the xor exchanges with stride 1, 2 (quad_perm) and
-// 8 (row_ror:8).  Other widths and strides use __shfl_up / __shfl_xor.  The
+// Shuffles that stay within one 16-lane DPP row are DPP moves, which stay in
+// the VALU instead of a round trip through the LDS crossbar: in a group of 16
+// lanes the shift up by one (row_shr:1), and in groups of 16 or more lanes the
+// xor exchanges with stride 1, 2 (quad_perm) and 8 (row_ror:8).  Other widths
+// and strides use __shfl_up / __shfl_xor.  The
 // moves carry no old value (bound_ctrl: a lane without a source reads 0), so
 // each is one v_mov_b32_dpp with no copy in front; the only such lane is lane 0
 // of a row under row_shr:1, and its shifted value is never used (a k list's
@@ -326,7 +330,7 @@ __device__ __forceinline__ T shfl_up1(T x) {
 }
 template <int G, typename T>
 __device__ __forceinline__ T shfl_xor_g(T x, uint32_t stride) {
-  if constexpr (G == 16) {
+  if constexpr (G >= 16) {
     if (stride == 1 || stride == 2 || stride == 8) {
       if constexpr (sizeof(T) == 8) {
         return stride == 1 ? dpp64<0xB1>(x) : stride == 2 ? dpp64<0x4E>(x) : dpp64<0x128>(x);
@@ -634,7 +638,7 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
         const uint32_t myidx = hit ? a.idx[start + o] : 0u;
         if constexpr (List::kBulk) {
           const uint32_t nv = (uint32_t)__popcll((uint64_t)gm);
-          if (nv >= BULK_MIN) {
+          if (nv >= bulk_min<G>()) {
             bound = best.insert_many(dist, myidx, pending, nv, bound, lane, a);
             if constexpr (SCREEN) t32 = screen_threshold(bound, screen_e, a);
             return;

@@ -248,3 +248,33 @@ def test_lanes_per_query_do_not_change_results(group, dims, k):
     assert rc == 0, tree._lib.kdt_last_error().decode()
     np.testing.assert_array_equal(out_i, oi)
     assert out_d.tobytes() == od.tobytes()
+
+
+@pytest.mark.parametrize("group", [8, 16, 32, 64])
+@pytest.mark.parametrize("case", ["lattice", "tiny"])
+def test_merge_ties_and_padding_every_group_width(group, case):
+    # The k list's compare-exchange steps decide by one lexicographic test
+    # (distance, then original index); exact ties, duplicate points and the
+    # padding entries of a list with fewer than k points all reach it.  k equals
+    # the group width, leaves of 32 rows give up to 32 hits per chunk, so both
+    # the bulk merge and the single inserts run at every width.
+    import ctypes
+
+    g = np.stack(np.meshgrid(np.arange(10.0), np.arange(10.0), np.arange(5.0)), -1).reshape(-1, 3)
+    data = np.concatenate([g, g[::3], g[::11]]) if case == "lattice" else g[: group // 2 + 3]
+    rng = np.random.default_rng(200 + group)
+    queries = rng.integers(0, 10, size=(2500, 3)).astype(float) + 0.5 * rng.integers(0, 2, size=(2500, 3))
+    k = group
+    od, oi = OracleTree(data, 32).query(queries, k=k)
+    tree = KDTree(data, leafsize=32)
+    code = {8: 1, 16: 2, 32: 3, 64: 4}[group]
+    out_d = np.empty((queries.shape[0], k))
+    out_i = np.empty((queries.shape[0], k), dtype=np.int64)
+    rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(queries.ctypes.data), queries.shape[0], 3, k, 2.0,
+                             np.inf, 0.0, code << 4, ctypes.c_void_p(out_d.ctypes.data),
+                             ctypes.c_void_p(out_i.ctypes.data))
+    assert rc == 0, tree._lib.kdt_last_error().decode()
+    if case == "tiny":
+        assert (oi == data.shape[0]).any()  # padded rows are part of the check
+    np.testing.assert_array_equal(out_i, oi)
+    assert out_d.tobytes() == od.tobytes()

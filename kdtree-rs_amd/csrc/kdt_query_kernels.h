@@ -57,9 +57,11 @@ namespace kdt {
 // STK_MAX otherwise; the runtime-width kernel keeps STK_MAX.
 constexpr int STK_MAX = 64;
 
-// Waves per SIMD the register-list kernels are compiled for.  The descent is
-// bound by the latency of its dependent node loads more than by issue, so
-// occupancy pays as long as the hot loops keep their registers: rows of 1..4
+// Waves per SIMD the register-list kernels are compiled for.  The descent's
+// dependent node loads need many waves to hide them; with enough waves the
+// traversal is bound by VALU issue (3-d rows at 8 waves: about 90% of the
+// SIMDs' VALU cycles), so occupancy pays as long as the hot loops keep their
+// registers: rows of 1..4
 // coordinates run 8 waves (a few registers of the query start-up spill), rows
 // of 8 run 6 (7 or 8 spill in the hot loops), rows of 16 take the registers
 // they need (120: 4 waves; at 5 or 6 waves their leaf loop spills).

@@ -61,10 +61,10 @@ constexpr int STK_MAX = 64;
 // dependent node loads need many waves to hide them; with enough waves the
 // traversal is bound by VALU issue (3-d rows at 8 waves: about 90% of the
 // SIMDs' VALU cycles), so occupancy pays as long as the hot loops keep their
-// registers: rows of 1..4
-// coordinates run 8 waves (a few registers of the query start-up spill), rows
-// of 8 run 6 (7 or 8 spill in the hot loops), rows of 16 take the registers
-// they need (120: 4 waves; at 5 or 6 waves their leaf loop spills).
+// registers: rows of 1..4 coordinates run 8 waves (a few registers of the
+// query start-up spill), rows of 8 run 6 (7 or 8 spill in the hot loops), rows
+// of 16 take the registers they need (120: 4 waves; at 5 or 6 waves their leaf
+// loop spills).
 template <int DT>
 constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : DT == 8 ? 6 : 1; }
 // Hits in one leaf chunk from which a group inserts them with one bitonic
@@ -612,7 +612,10 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
           }
         }
       }
-      if (!__ballot(alive && state == ST_DESCEND)) break;
+      // A group without a query stays in ST_DONE (state changes only under
+      // `alive`), so `state == X` alone is the phase test: a ballot of one
+      // compare is its lane mask, with no 0/1 value formed and re-tested.
+      if (!__ballot(state == ST_DESCEND)) break;
     }
     // 2. scan_leaf, src/query.rs:343-353; leaf scans, src/kernel.rs:359-563
     if (alive && state == ST_SCAN) {
@@ -712,7 +715,7 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
           state = ST_DESCEND;
         }
       }
-      if (!__ballot(alive && state == ST_POP)) break;
+      if (!__ballot(state == ST_POP)) break;
     }
     // 4. finished queries: results out, next query in
     if (alive && state == ST_DONE) {

@@ -1,18 +1,34 @@
-"""Benchmark: exact k-NN queries/s of the plain GPU kernel on one MI355X.
+"""Benchmark: served vs plain GPU exact k-NN on one MI355X.
 
-Workload (the plain GPU kernel configuration): a KD-tree over n = 2^20 points,
-d = 3, float64 uniform on [0,1)^3, leafsize 32, queried with 2^20 uniform
-queries, k = 16, p = 2.  One step = one query pass over the whole batch, with
-points, tree and queries resident in HBM.  Results are exact (bit-identical to
-the oracle) and a sample is checked each run.
+Default workload (the serving configuration): 32 resident trees (n
+log-uniform in [1e5, 8e6], d alternating 16 / 64, float32 standard normal
+promoted to float64), Poisson arrivals of query requests on uniform trees
+with batch sizes log-uniform in [1, 65536], k = 10, 200 GB HBM admission
+budget.  One step = one serving run of the 240-request stream, arriving as a
+Poisson process at the server's measured capacity (requests per second of a
+saturated run), through the continuous-batching loop (kdtree.serve) with the
+FP32 matrix-core screens and FP64 recheck.  `value` = served queries/s over
+the timed steps, with p50/p99 latency (arrival to host-visible result).
 
-Prints one JSON line (rank 0).  `roofline` uses the algorithmic bytes per
-query from the reference traversal counters (leaf rows scanned, nodes visited,
-box tests, measured with the oracle on 10^4 queries):
-    B_q = 8d + 16k + E(8d + 4) + 24V + 16dX
-and the query kernel's average duration from HIP events on its stream.
-`cpu_baseline` times the oracle (the C++ restatement of the reference's query,
-threaded like its rayon path) on this host's cores.
+Beside it, on the same requests: the plain GPU kernel (each request alone,
+queries resident in HBM, one kdt_query_device call with the FP64 traversal)
+on a random sample of the stream, its queries/s and the served/plain ratio;
+and a byte-for-byte comparison of the two.  `roofline` is the dominant
+kernel, the 64-d matrix-core screen (k_screen<32>), timed live by HIP events
+around each of its launches in the timed region, against the dense bf16
+matrix peak, with 2 d FLOPs per (query, point) pair as the algorithmic work.
+`cpu_baseline` times the oracle (the C++ restatement of the reference's
+query, threaded like its rayon path) on a bounded sample of the same requests
+on this host's cores: 1 warm-up, then the median and minimum of 3 repeats.
+
+Nested in the same line: the precision workload (n = 2^22 8-d Gaussian
+mixture, 10^6 queries, k = 32: FP64 vs FP32-screened traversal, with its HBM
+roofline and CPU ratio) and the plain GPU kernel workload (n = 2^20 uniform
+3-d points, 2^20 queries, k = 16, with its HBM roofline).
+
+Other workloads (--workload): c2, c3 (those lines alone), c4 (256-tree
+batching), screen (the matrix-core screen alone on one 64-d tree), copy (a
+1 GiB device copy that calibrates the PMC byte counters).
 """
 
 from __future__ import annotations
@@ -21,8 +37,10 @@ import argparse
 import ctypes
 import json
 import os
+import subprocess
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -32,10 +50,9 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "kdtree-rs_amd"))
 
 from kdtree._core import load_library  # noqa: E402
-from kdtree.workloads import C2, c2_inputs  # noqa: E402
 
-KDT_QUERY_ASSUME_FINITE = 0x2
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # dense bf16 matrix-core peak
 
 
 def parse():
@@ -43,19 +60,31 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU baseline sample time")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5", "screen", "copy"],
-                    help="c2 (default, the metric's configuration); c3 precision mode; c4 multi-tree batch; "
-                         "c5 serving; screen: the matrix-core screen alone on one 64-d tree; copy: a 1 GiB "
-                         "device copy that calibrates the FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--requests", type=int, default=240, help="c5: requests per serving run")
-    ap.add_argument("--plain-seconds", type=float, default=40.0, help="c5: time budget of the plain sample")
+    ap.add_argument("--workload", default="serve", choices=["serve", "c2", "c3", "c4", "screen", "copy"],
+                    help="serve (default: the serving configuration, with the precision and plain GPU kernel "
+                         "workloads nested); c2 plain GPU kernel; c3 precision; c4 multi-tree batch; screen: the "
+                         "matrix-core screen alone on one 64-d tree; copy: a 1 GiB device copy that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--requests", type=int, default=240, help="serve: requests per serving run")
+    ap.add_argument("--plain-seconds", type=float, default=25.0, help="serve: time budget of the plain sample")
+    ap.add_argument("--no-nested", action="store_true", help="serve: skip the nested precision / plain lines")
     return ap.parse_args()
 
 
+def log(msg: str):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_model() -> str:
-    """The host CPU's model name (first entry of /proc/cpuinfo), or "unknown"."""
+    """The host CPU's model name (lscpu, else /proc/cpuinfo), or "unknown"."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
             if line.startswith("model name"):
@@ -65,194 +94,21 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def traffic_from_profiles():
-    """HBM bytes per query launch from the committed rocprofv3 PMC summary, if any."""
-    f = ROOT / "profiles" / "knn_c2_pmc.md"
-    if not f.exists():
-        return None
-    for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+def host_threads() -> int:
+    return len(os.sched_getaffinity(0))
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the workload is a single-GPU "
-              "query pass and no multi-GPU path is built", file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "c3":
-        return run_c3(args)
-    if args.workload == "c5":
-        return run_c5(args)
-    if args.workload == "screen":
-        return run_screen(args)
-    if args.workload == "copy":
-        return run_copy(args)
-    if args.workload == "c4":
-        return run_c4(args)
-    import torch
-
-    lib = load_library()
-    dev = torch.device("cuda:0")
-    n, nq, d, k, ls = C2["n"], C2["nq"], C2["d"], C2["k"], C2["leafsize"]
-    data_h, q_h = c2_inputs()
-    data = torch.from_numpy(data_h).to(dev)
-    q = torch.from_numpy(q_h).to(dev)
-    out_d = torch.empty((nq, k), dtype=torch.float64, device=dev)
-    out_i = torch.empty((nq, k), dtype=torch.int64, device=dev)
-    stream = torch.cuda.current_stream()
-    sp = ctypes.c_void_p(stream.cuda_stream)
-    torch.cuda.synchronize()
-
-    # Build (timed separately; not part of the metric).
-    tree = ctypes.c_void_p()
-    t0 = time.perf_counter()
-    rc = lib.kdt_build_device(ctypes.c_void_p(data.data_ptr()), n, d, ls, sp, ctypes.byref(tree))
-    torch.cuda.synchronize()
-    build_ms = (time.perf_counter() - t0) * 1e3
-    if rc:
-        raise RuntimeError(lib.kdt_last_error().decode())
-
-    def query(flags):
-        r = lib.kdt_query_device(tree, ctypes.c_void_p(q.data_ptr()), nq, d, k, 2.0, float("inf"), 0.0, flags,
-                                 ctypes.c_void_p(out_d.data_ptr()), ctypes.c_void_p(out_i.data_ptr()), sp)
-        if r:
-            raise RuntimeError(lib.kdt_last_error().decode())
-
-    query(0)  # validates the queries (finiteness) once
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        query(KDT_QUERY_ASSUME_FINITE)
-    torch.cuda.synchronize()
-
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
-        query(KDT_QUERY_ASSUME_FINITE)
-        ends[i].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = np.array([s.elapsed_time(e) for s, e in zip(starts, ends)])
-    qps = nq * args.steps / wall
-
-    # The drop-in boundary hands over host buffers: the same queries through
-    # kdt_query (queries host -> device, finiteness check, query, results
-    # device -> host), PCIe included.  Reported beside the metric, never as it.
-    hd = np.empty((nq, k), np.float64)
-    hi = np.empty((nq, k), np.int64)
-
-    def host_query():
-        r = lib.kdt_query(tree, ctypes.c_void_p(q_h.ctypes.data), nq, d, k, 2.0, float("inf"), 0.0, 0,
-                          ctypes.c_void_p(hd.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
-        if r:
-            raise RuntimeError(lib.kdt_last_error().decode())
-
-    host_query()
-    t0 = time.perf_counter()
-    host_reps = 3
-    for _ in range(host_reps):
-        host_query()
-    host_s = (time.perf_counter() - t0) / host_reps
-    host_same = bool(np.array_equal(hi, out_i.cpu().numpy()) and hd.tobytes() == out_d.cpu().numpy().tobytes())
-
-    # Parity sample against the oracle (the checker).
-    from oracle import OracleTree
-
-    sample = 2000
-    t0 = time.perf_counter()
-    otree = OracleTree(data_h, ls)
-    cpu_build_ms = (time.perf_counter() - t0) * 1e3  # the oracle's build: one thread
-    od, oi, ctr = otree.query(q_h[:10_000], k=k, counters=True)
-    gd = out_d[:sample].cpu().numpy()
-    gi = out_i[:sample].cpu().numpy()
-    parity = bool(np.array_equal(gi, oi[:sample]) and gd.tobytes() == od[:sample].tobytes())
-
-    E, V, X = (ctr / 10_000.0).tolist()
-    bq = 8 * d + 16 * k + E * (8 * d + 4) + 24 * V + 16 * d * X
-    avg_kernel_s = float(kernel_ms.mean()) / 1e3
-    achieved = bq * nq / avg_kernel_s / 1e9
-    traffic = traffic_from_profiles()
-
-    cpu = None
-    if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        probe = q_h[:20_000]
+def cpu_timed(fn, units: float, reps: int = 3) -> dict:
+    """The reference's timing protocol (benches/grid.rs:367-388): one warm-up,
+    then `reps` timed repeats; value = units / median time."""
+    fn()
+    ts = []
+    for _ in range(reps):
         t0 = time.perf_counter()
-        otree.query(probe, k=k, threads=threads)
-        per_q = (time.perf_counter() - t0) / probe.shape[0]
-        m = int(min(nq, max(20_000, args.cpu_seconds / max(per_q, 1e-9))))
-        t0 = time.perf_counter()
-        otree.query(q_h[:m], k=k, threads=threads)
-        cpu_qps = m / (time.perf_counter() - t0)
-        m1 = int(min(nq, max(2_000, 2.0 / max(per_q * threads, 1e-9))))  # about 2 s on one core
-        t0 = time.perf_counter()
-        otree.query(q_h[:m1], k=k, threads=1)
-        one_core = m1 / (time.perf_counter() - t0)
-        cpu = {
-            "value": cpu_qps,
-            "unit": "queries/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"{m} of the {nq} queries, k={k}, on the same tree, {threads} threads "
-                      "(oracle = C++ restatement of the reference query, g++ -O3 -ffp-contract=off)",
-            "one_core": {"value": one_core, "sample": f"{m1} queries, 1 thread"},
-            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
-        }
-        try:  # secondary: SciPy's cKDTree (the reference's own tests compare against SciPy)
-            import scipy
-            from scipy.spatial import cKDTree
-        except ImportError:
-            scipy = None
-        if scipy is not None:
-            t0 = time.perf_counter()
-            stree = cKDTree(data_h, leafsize=ls)
-            sbuild = time.perf_counter() - t0
-            t0 = time.perf_counter()
-            stree.query(q_h[:m], k=k, workers=threads)
-            cpu["scipy_ckdtree"] = {"value": m / (time.perf_counter() - t0), "unit": "queries/s",
-                                    "workers": threads, "build_ms": sbuild * 1e3, "version": scipy.__version__,
-                                    "sample": f"the same {m} queries and points, leafsize {ls}"}
-
-    line = {
-        "metric": "k-NN queries/s (exact indices), plain GPU kernel",
-        "value": qps,
-        "unit": "queries/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (SplitMix64 uniform, seeds 2 / 1002)",
-        "config": {"workload": "knn-uniform-n1048576-d3-q1048576-k16-leafsize32-p2", "n_points": n,
-                   "n_queries": nq, "ndim": d, "k": k, "leafsize": ls},
-        "latency_ms": {"p50": float(np.percentile(kernel_ms, 50)), "p99": float(np.percentile(kernel_ms, 99)),
-                       "what": "one full 2^20-query batch"},
-        "build_ms": build_ms,
-        "build_vs_cpu": {"gpu_ms": build_ms, "cpu_ms": cpu_build_ms, "cpu_threads": 1, "kind": "port",
-                         "what": "build of the 2^20-point tree: oracle on one host thread (the build parity "
-                                 "tests check that both trees are identical); GPU time includes its allocations"},
-        "parity_sample_bit_exact": parity,
-        "host_buffers_pcie_inclusive": {"queries_per_s": nq / host_s, "ms_per_call": host_s * 1e3,
-                                        "same_as_device_path": host_same,
-                                        "what": "kdt_query on pageable host arrays: 24 MiB of queries in, "
-                                                "256 MiB of results out, finiteness check included"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_query": bq, "E": E, "V": V, "X": X,
-                     "kernel_ms_avg": float(kernel_ms.mean())},
-        "distance_evals_per_s": E * nq / avg_kernel_s,
-        "cpu_baseline": cpu,
-    }
-    lib.kdt_free(tree)
-    print(json.dumps(line), flush=True)
+        fn()
+        ts.append(time.perf_counter() - t0)
+    med, mn = float(np.median(ts)), float(min(ts))
+    return {"value": units / med, "best": units / mn, "median_s": med, "min_s": mn, "repeats": reps}
 
 
 def _time_steps(fn, steps, warmup):
@@ -268,7 +124,100 @@ def _time_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def run_c3(args):
+def traversal_bytes(d: int, k: int, E: float, V: float, X: float) -> float:
+    """Algorithmic bytes per query over the reference traversal: the query,
+    the results, E leaf rows scanned (coordinates + index), V nodes visited,
+    X box tests."""
+    return 8 * d + 16 * k + E * (8 * d + 4) + 24 * V + 16 * d * X
+
+
+def kernel_summary(times: dict) -> dict:
+    return {name: {"launches": v["launches"], "ms_total": v["ms"], "ms_avg": v["ms"] / max(1, v["launches"]),
+                   "work": v["work"]} for name, v in sorted(times.items(), key=lambda kv: -kv[1]["ms"])}
+
+
+# --------------------------------------------------------------- plain GPU kernel
+def c2_line(args, steps: int, warmup: int, cpu: bool) -> dict:
+    """Plain GPU kernel: n = 2^20 uniform 3-d points, 2^20 queries, k = 16.
+    One step = one kdt_query_device call over the resident queries, with its
+    finiteness check (the reference checks every call, src/query.rs:46)."""
+    import torch
+
+    from kdtree import forest as F
+    from kdtree.workloads import C2, c2_inputs
+    from oracle import OracleTree
+
+    lib = load_library()
+    n, nq, d, k, ls = C2["n"], C2["nq"], C2["d"], C2["k"], C2["leafsize"]
+    data_h, q_h = c2_inputs()
+    data = torch.from_numpy(data_h).cuda()
+    q = torch.from_numpy(q_h).cuda()
+    out_d = torch.empty((nq, k), dtype=torch.float64, device="cuda")
+    out_i = torch.empty((nq, k), dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tree = F.build(data, ls)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - t0) * 1e3
+
+    def query():
+        r = lib.kdt_query_device(tree.handle, ctypes.c_void_p(q.data_ptr()), nq, d, k, 2.0, float("inf"), 0.0, 0,
+                                 ctypes.c_void_p(out_d.data_ptr()), ctypes.c_void_p(out_i.data_ptr()), sp)
+        if r:
+            raise RuntimeError(lib.kdt_last_error().decode())
+
+    for _ in range(warmup):
+        query()
+    torch.cuda.synchronize()
+    F.timing_enable(True)
+    F.timing_read()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        query()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    F.timing_enable(False)
+    times = F.timing_read()
+    trav = max(times.items(), key=lambda kv: kv[1]["ms"])
+    kernel_ms = trav[1]["ms"] / trav[1]["launches"]
+
+    otree = OracleTree(data_h, ls)
+    od, oi, ctr = otree.query(q_h[:10_000], k=k, counters=True, threads=host_threads())
+    sample = 2000
+    parity = bool(np.array_equal(out_i[:sample].cpu().numpy(), oi[:sample])
+                  and out_d[:sample].cpu().numpy().tobytes() == od[:sample].tobytes())
+    E, V, X = (ctr / 10_000.0).tolist()
+    bq = traversal_bytes(d, k, E, V, X)
+    achieved = bq * nq / (kernel_ms / 1e3) / 1e9
+    line = {
+        "workload": "knn-uniform-n1048576-d3-q1048576-k16-leafsize32-p2",
+        "value": nq * steps / wall, "unit": "queries/s", "steps": steps, "ms_per_step": wall / steps * 1e3,
+        "dtype": "f64", "build_ms": build_ms, "parity_sample_bit_exact_vs_oracle": parity,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": trav[0],
+                     "kernel_ms_avg": kernel_ms, "bytes_per_query": bq, "E": E, "V": V, "X": X,
+                     "what": "algorithmic bytes of the reference traversal per query (query, results, E leaf rows, "
+                             "V nodes, X box tests; oracle counters on 10^4 queries) x 2^20 queries / the traversal "
+                             "kernel's HIP-event time"},
+        "distance_evals_per_s": E * nq / (kernel_ms / 1e3),
+        "kernel_times": kernel_summary(times),
+    }
+    if cpu:
+        threads = host_threads()
+        m = 200_000
+        c = cpu_timed(lambda: otree.query(q_h[:m], k=k, threads=threads), m)
+        c.update(unit="queries/s", cores=threads, kind="port",
+                 sample=f"{m} of the {nq} queries, k={k}, same tree, {threads} threads")
+        line["cpu_baseline"] = c
+        line["vs_cpu"] = line["value"] / c["value"]
+    tree.free()
+    return line
+
+
+# --------------------------------------------------------------- precision mode
+def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     """Precision mode: n = 2^22, d = 8, 64-component Gaussian mixture, 10^6
     queries, k = 32; all-FP64 leaf scan versus FP32 screen + FP64 recheck."""
     import torch
@@ -278,7 +227,7 @@ def run_c3(args):
     from oracle import OracleTree
 
     data_h, q_h = c3_inputs()
-    k, ls = C3["k"], C3["leafsize"]
+    k, ls, d = C3["k"], C3["leafsize"], C3["d"]
     data = torch.from_numpy(data_h).cuda()
     q = torch.from_numpy(q_h).cuda()
     torch.cuda.synchronize()
@@ -289,39 +238,292 @@ def run_c3(args):
     nq = q.shape[0]
     od64, oi64 = F.query_device(tree, q, k, flags=0)
     od32, oi32 = F.query_device(tree, q, k, flags=F.KDT_QUERY_FP32_SCREEN)
-    s64 = _time_steps(lambda: F.query_device(tree, q, k, od64, oi64), args.steps, args.warmup)
-    s32 = _time_steps(lambda: F.query_device(tree, q, k, od32, oi32,
-                                             flags=F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN),
-                      args.steps, args.warmup)
+    F.timing_enable(True)
+    F.timing_read()
+    s64 = _time_steps(lambda: F.query_device(tree, q, k, od64, oi64, flags=0), steps, warmup)
+    t64 = F.timing_read()
+    s32 = _time_steps(lambda: F.query_device(tree, q, k, od32, oi32, flags=F.KDT_QUERY_FP32_SCREEN), steps, warmup)
+    t32 = F.timing_read()
+    F.timing_enable(False)
     same_idx = bool(torch.equal(oi64, oi32))
+    same_bytes = bool(torch.equal(od64, od32))
     d64, d32 = od64.cpu().numpy(), od32.cpu().numpy()
     rel = float(np.max(np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-300)))
-    sample = 2000
-    t0 = time.perf_counter()
     otree = OracleTree(data_h, ls)
-    cpu_build_ms = (time.perf_counter() - t0) * 1e3  # the oracle's build: one thread
-    od, oi = otree.query(q_h[:sample], k=k)
-    parity = bool(np.array_equal(oi, oi64[:sample].cpu().numpy()) and od.tobytes() == d64[:sample].tobytes())
+    od, oi, ctr = otree.query(q_h[:10_000], k=k, counters=True, threads=host_threads())
+    sample = 2000
+    parity = bool(np.array_equal(oi[:sample], oi64[:sample].cpu().numpy())
+                  and od[:sample].tobytes() == d64[:sample].tobytes())
+    E, V, X = (ctr / 10_000.0).tolist()
+    bq = traversal_bytes(d, k, E, V, X)
+
+    def roof(times):
+        name, v = max(times.items(), key=lambda kv: kv[1]["ms"])
+        ms = v["ms"] / v["launches"]
+        ach = bq * nq / (ms / 1e3) / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                "traffic": None, "kernel": name, "kernel_ms_avg": ms}
+
+    line = {
+        "workload": "knn-gmm64-n4194304-d8-q1000000-k32-leafsize32-p2",
+        "fp64": {"value": nq / s64, "ms_per_step": s64 * 1e3, "roofline": roof(t64)},
+        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3, "roofline": roof(t32)},
+        "unit": "queries/s", "steps": steps, "bytes_per_query": bq, "E": E, "V": V, "X": X,
+        "distance_evals_per_s_fp64": E * nq / s64,
+        "distance_flops_per_s_fp64": 3 * d * E * nq / s64,
+        "indices_identical_fp32_vs_fp64": same_idx, "distances_identical_fp32_vs_fp64": same_bytes,
+        "max_rel_distance_error_fp32_vs_fp64": rel,
+        "parity_sample_bit_exact_vs_oracle": parity, "build_ms": build_ms,
+    }
+    if cpu:
+        threads = host_threads()
+        m = 40_000
+        c = cpu_timed(lambda: otree.query(q_h[:m], k=k, threads=threads), m)
+        c.update(unit="queries/s", cores=threads, kind="port", sample=f"{m} of the {nq} queries, {threads} threads")
+        line["cpu_baseline"] = c
+        line["vs_cpu_fp64"] = line["fp64"]["value"] / c["value"]
+        line["vs_cpu_fp32_screen"] = line["fp32_screen"]["value"] / c["value"]
+    tree.free()
+    return line
+
+
+# --------------------------------------------------------------- serving
+def c5_points(t: int, n: int, d: int):
+    import torch
+
+    from kdtree.workloads import C5
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(C5["seed"] * 1000 + t)
+    return torch.randn((n, d), generator=g, device="cuda", dtype=torch.float32).double()
+
+
+def run_serve(args):
+    import torch
+
+    from kdtree import KDTree  # noqa: F401  (loads the library with torch's runtime first)
+    from kdtree import forest as F
+    from kdtree.serve import Request, Server, latency_stats, poisson_requests
+    from kdtree.workloads import C5, c5_queries, c5_specs
+    from oracle import OracleTree
+
+    specs = c5_specs()
+    dims = [d for _, d in specs]
+    k = C5["k"]
+    log("building 32 trees")
+    trees = []
+    t0 = time.perf_counter()
+    for t, (n, d) in enumerate(specs):
+        x = c5_points(t, n, d)
+        trees.append(F.build(x, 32))
+        del x
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+
+    # d = 16: matrix-core screen or FP32-screened traversal, whichever is
+    # faster on a sample (d = 64 always takes the matrix-core screen).
+    t16 = sorted([t for t in range(len(specs)) if dims[t] == 16], key=lambda t: specs[t][0])
+    t16 = t16[len(t16) // 2]
+    qs = torch.from_numpy(c5_queries(10**6, 4096, 16)).cuda()
+    tune = {}
+    for name, fl in (("traversal_fp32_screen", F.KDT_QUERY_FP32_SCREEN),
+                     ("matrix_screen", F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_MATRIX_SCREEN)):
+        F.query_device(trees[t16], qs, k, flags=fl | F.KDT_QUERY_ASSUME_FINITE)
+        tune[name] = _time_steps(lambda: F.query_device(trees[t16], qs, k, flags=fl | F.KDT_QUERY_ASSUME_FINITE),
+                                 3, 1) * 1e3
+    serve_flags = F.KDT_QUERY_FP32_SCREEN | (
+        F.KDT_QUERY_MATRIX_SCREEN if tune["matrix_screen"] < tune["traversal_fp32_screen"] else 0)
+
+    base = poisson_requests(args.requests, rate=1.0, tree_dims=dims, k=k, max_batch=C5["max_batch"],
+                            seed=C5["seed"], make_queries=c5_queries)
+    total_q = sum(r.queries.shape[0] for r in base)
+
+    def fresh(rate, subset=None):
+        src = base if subset is None else subset
+        return [Request(r.rid, r.arrival / rate, r.tree, r.queries, r.k) for r in src]
+
+    def serve(reqs, stats=True):
+        server = Server(trees, n_streams=4, hbm_budget_bytes=C5["hbm_budget_bytes"], flags=serve_flags)
+        server.run(reqs, time_limit=600)
+        st = latency_stats(reqs) if stats else {}
+        st.update(launches=server.launches, max_inflight=server.max_inflight)
+        return st
+
+    # Capacity: the whole stream waiting at once (also the warm-up of every
+    # kernel and of the caching allocator).
+    log("capacity run")
+    sat = serve(fresh(1e12))
+    cap_rps = sat["requests"] / sat["span_s"]
+    for _ in range(args.warmup):
+        serve(fresh(cap_rps), stats=False)
+    torch.cuda.synchronize()
+
+    # Timed steps: the stream at Poisson arrivals, offered at the capacity.
+    log(f"timed steps at {cap_rps:.1f} requests/s")
+    F.timing_enable(True)
+    F.timing_read()
+    runs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reqs = fresh(cap_rps)
+        serve(reqs, stats=False)
+        runs.append(reqs)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    F.timing_enable(False)
+    times = F.timing_read()
+    lat = np.concatenate([[r.done - r.arrival for r in rs] for rs in runs]) * 1e3
+    served_qps = total_q * args.steps / wall
+    scr = times.get("k_screen<32>", {"launches": 0, "ms": 0.0, "work": 0.0})
+    achieved = scr["work"] / (scr["ms"] / 1e3) / 1e12 if scr["ms"] else 0.0
+
+    # Latency at lower loads.
+    loads = {}
+    for frac in (0.5, 0.8):
+        loads[f"{frac:.1f}x_capacity"] = serve(fresh(cap_rps * frac))
+
+    # The screen alone: 65,536 queries on the largest 64-d tree, one stream.
+    big = max((t for t in range(len(specs)) if dims[t] == 64), key=lambda t: specs[t][0])
+    qa = torch.from_numpy(c5_queries(2 * 10**6, 65536, 64)).cuda()
+    fl = F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_ASSUME_FINITE
+    F.query_device(trees[big], qa, k, flags=fl)
+    torch.cuda.synchronize()
+    F.timing_enable(True)
+    F.timing_read()
+    alone_s = _time_steps(lambda: F.query_device(trees[big], qa, k, flags=fl), 3, 0)
+    alone_t = F.timing_read()
+    F.timing_enable(False)
+    sa = alone_t.get("k_screen<32>", {"launches": 1, "ms": float("nan"), "work": 0.0})
+    alone_tf = sa["work"] / (sa["ms"] / 1e3) / 1e12
+    del qa
+
+    # Plain GPU kernel on a random sample of the same requests: each alone,
+    # queries already in HBM, one kdt_query_device call (FP64 traversal).
+    log("plain sample")
+    rng = np.random.default_rng(7)
+    plain, plain_t, plain_q = [], 0.0, 0
+    last = {r.rid: r for r in runs[-1]}
+    for j in rng.permutation(len(base)):
+        r = base[int(j)]
+        qd = torch.from_numpy(r.queries).cuda()
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        pd, pi = F.query_device(trees[r.tree], qd, k, flags=0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - s0
+        plain.append((r, dt, pd.cpu().numpy(), pi.cpu().numpy()))
+        plain_t += dt
+        plain_q += r.queries.shape[0]
+        if plain_t > args.plain_seconds and len(plain) >= 8:
+            break
+    plain_qps = plain_q / plain_t
+    plain_lat = np.array([dt for _, dt, _, _ in plain]) * 1e3
+    exact = all(np.array_equal(last[r.rid].out_i, pi) and last[r.rid].out_d.tobytes() == pd.tobytes()
+                for r, _, pd, pi in plain)
+    sample = [r for r, _, _, _ in plain]
+    same = fresh(1e12, sample)
+    served_sample = serve(same)
+    exact_sample = all(np.array_equal(s.out_i, pi) and s.out_d.tobytes() == pd.tobytes()
+                       for s, (_, _, pd, pi) in zip(same, plain))
+
+    # Oracle: two requests of the sample on its smallest trees, 16 queries each.
+    parity, checked = True, 0
+    for r, _, pd, pi in sorted(plain, key=lambda x: specs[x[0].tree][0])[:2]:
+        host = c5_points(r.tree, *specs[r.tree]).cpu().numpy()
+        m = min(16, r.queries.shape[0])
+        od, oi = OracleTree(host, 32).query(r.queries[:m], k=k, threads=host_threads())
+        parity &= bool(np.array_equal(oi, pi[:m]) and od.tobytes() == pd[:m].tobytes())
+        checked += 1
+
     cpu = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
-        m = 20_000
-        t0 = time.perf_counter()
-        otree.query(q_h[:m], k=k, threads=threads)
-        cpu = {"value": m / (time.perf_counter() - t0), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"{m} of the {nq} queries, k={k}, {threads} threads (oracle)"}
-    F_tree = tree
+        # The oracle on the plain sample's requests, each cut to 32 queries,
+        # over the trees they use (built beforehand in parallel, not timed).
+        log("cpu baseline")
+        threads = host_threads()
+        cut = [(r.tree, np.ascontiguousarray(r.queries[:32])) for r in sample]
+        need = sorted({t for t, _ in cut})
+
+        def build_oracle(t):
+            host = c5_points(t, *specs[t]).cpu().numpy()
+            return t, OracleTree(host, 32)
+
+        with ThreadPoolExecutor(max_workers=min(8, len(need))) as ex:
+            otrees = dict(ex.map(build_oracle, need))
+        units = sum(q.shape[0] for _, q in cut)
+        cpu = cpu_timed(lambda: [otrees[t].query(q, k=k, threads=threads) for t, q in cut], units)
+        cpu.update(unit="queries/s", cores=threads, kind="port", cpu_model=cpu_model(), host_cpus=os.cpu_count(),
+                   sample=f"{len(cut)} requests of the plain sample on {len(need)} trees, at most 32 queries each "
+                          f"({units} queries), k={k}, {threads} threads (oracle = C++ restatement of the reference "
+                          "query, g++ -O3 -ffp-contract=off)")
+        del otrees
+
     out = {
-        "metric": "k-NN queries/s (exact indices), precision mode",
-        "config": {"workload": "knn-gmm64-n4194304-d8-q1000000-k32-leafsize32-p2"},
-        "fp64": {"value": nq / s64, "ms_per_step": s64 * 1e3},
-        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3},
-        "indices_identical": same_idx, "max_rel_distance_error_fp32_vs_fp64": rel,
-        "parity_sample_bit_exact_vs_oracle": parity, "build_ms": build_ms, "cpu_baseline": cpu,
-        "vs_cpu_fp32_screen": (nq / s32) / cpu["value"] if cpu else None,
+        "metric": "k-NN queries/s + p50/p99 latency, served vs plain GPU, exact indices",
+        "value": served_qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 screen (split bf16 matrix cores, f32 accumulate) + f64 recheck",
+        "data": "synthetic (torch.randn float32 points, seed 5000+t; SplitMix64 Box-Muller queries)",
+        "config": {"workload": "serve-32-trees-n1e5..8e6-d{16,64}-k10-batch1..65536-poisson-budget200GB",
+                   "n_trees": len(specs), "n_points": int(sum(n for n, _ in specs)), "requests": args.requests,
+                   "queries_per_run": int(total_q), "k": k, "offered_requests_per_s": cap_rps},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "what": "arrival to host-visible result, all requests of the timed steps"},
+        "plain": {"queries_per_s": plain_qps, "requests": len(plain), "queries": int(plain_q),
+                  "p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99)),
+                  "what": "a random sample of the same requests, each alone through kdt_query_device (FP64 "
+                          "traversal, queries resident in HBM), one after another; latency = its service time"},
+        "served_vs_plain": served_qps / plain_qps,
+        "served_same_sample": served_sample,
+        "served_vs_plain_same_sample": served_sample["queries_per_s"] / plain_qps,
+        "exact_served_vs_plain_bitwise": bool(exact and exact_sample),
+        "parity_vs_oracle_sample": parity, "oracle_checked_requests": checked,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_screen<32>",
+                     "launches": scr["launches"], "kernel_ms_avg": scr["ms"] / max(1, scr["launches"]),
+                     "flops_per_launch_avg": scr["work"] / max(1, scr["launches"]),
+                     "what": "2 d FLOPs per (query, point) pair of the 64-d matrix-core screen over the timed "
+                             "serving steps / the summed HIP-event durations of its launches (each on its own "
+                             "stream; concurrent launches share the device)"},
+        "roofline_screen_alone": {"achieved": alone_tf, "frac": alone_tf / BF16_PEAK_TFLOPS,
+                                  "kernel_ms_avg": sa["ms"] / max(1, sa["launches"]), "call_ms": alone_s * 1e3,
+                                  "what": "k_screen<32> on the largest 64-d tree, 65,536 queries, one stream"},
+        "kernel_times": kernel_summary(times),
+        "served_saturation": sat, "served_at_load": loads,
+        "d16_choice_ms_per_4096_queries": tune,
+        "d16_matrix_screen": bool(serve_flags & F.KDT_QUERY_MATRIX_SCREEN),
+        "build_s": build_s,
+        "cpu_baseline": cpu,
     }
-    F_tree.free()
+    if cpu:
+        out["vs_cpu"] = served_qps / cpu["value"]
+    for tr in trees:
+        tr.free()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if not args.no_nested:
+        log("nested: precision workload")
+        out["nested"] = {"precision": c3_line(args, 5, 1, not args.no_cpu)}
+        torch.cuda.empty_cache()
+        log("nested: plain GPU kernel workload")
+        out["nested"]["plain_gpu_kernel"] = c2_line(args, 10, 2, not args.no_cpu)
     print(json.dumps(out), flush=True)
+
+
+# --------------------------------------------------------------- other workloads
+def run_c2(args):
+    line = c2_line(args, args.steps, args.warmup, not args.no_cpu)
+    line.update(metric="k-NN queries/s (exact indices), plain GPU kernel", n_gpus=1, warmup=args.warmup,
+                higher_is_better=True, scaling="weak", vs_baseline=None,
+                data="synthetic (SplitMix64 uniform, seeds 2 / 1002)",
+                config={"workload": line.pop("workload")})
+    print(json.dumps(line), flush=True)
+
+
+def run_c3(args):
+    line = c3_line(args, args.steps, args.warmup, not args.no_cpu)
+    line.update(metric="k-NN queries/s (exact indices), precision mode", config={"workload": line.pop("workload")})
+    print(json.dumps(line), flush=True)
 
 
 def run_c4(args):
@@ -330,8 +532,7 @@ def run_c4(args):
     {1, 8, 32}; leafsize autotuned over {8,16,32,64}, lanes per query over the
     widths that hold k, and the matrix-core screen where it applies.  Shared
     persistent launches (items of 8192+ queries on their own) versus one launch
-    sequence per tree versus the per-tree launches captured as a HIP graph (a
-    pure enqueue: no leaf ordering and no matrix-core screen)."""
+    sequence per tree versus the batch plan and its HIP graph."""
     import torch
 
     from kdtree import forest as F
@@ -351,16 +552,16 @@ def run_c4(args):
     batch = F.Batch(items)
     nq = batch.n_queries
     res, outs = {}, {}
-    for mode in ("per_tree", "graph", "persistent"):
+    for mode in ("per_tree", "persistent", "plan", "graph"):
         res[mode] = nq / _time_steps(lambda: batch.run(mode), args.steps, args.warmup)
         outs[mode] = [(od.clone(), oi.clone()) for _, _, _, od, oi in batch.items]
     modes_equal = all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
-                      for m in ("per_tree", "graph") for a, b in zip(outs[m], outs["persistent"]))
+                      for m in ("per_tree", "plan", "graph") for a, b in zip(outs[m], outs["persistent"]))
     batch.run("persistent")
     torch.cuda.synchronize()
     parity = True
     for j, (data, queries, k, ls) in enumerate(host):
-        od, oi = OracleTree(data, ls).query(queries, k=k)
+        od, oi = OracleTree(data, ls).query(queries, k=k, threads=host_threads())
         gd, gi = batch.results(j)
         parity &= bool(np.array_equal(gi.cpu().numpy(), oi) and gd.cpu().numpy().tobytes() == od.tobytes())
     leafs, groups = {}, {}
@@ -373,11 +574,14 @@ def run_c4(args):
                    "n_points": int(sum(s[0] for s in specs))},
         "queries_per_s": res,
         "persistent_vs_per_tree": res["persistent"] / res["per_tree"],
+        "graph_vs_persistent": res["graph"] / res["persistent"],
+        "plan_bytes": int(F._lib().kdt_batch_plan_bytes(batch.plan())),
         "autotuned_leafsize_counts": leafs, "autotuned_lanes_per_query_counts": groups, "autotune_s": tune_s,
         "parity_first_8_trees_bit_exact": parity,
         "all_items_bitwise_equal_across_modes": bool(modes_equal),
         "matrix_screen_trees": sum(1 for tree, _, _ in items if tree.flags),
     }
+    batch.close()
     for tree, _, _ in items:
         tree.free()
     print(json.dumps(out), flush=True)
@@ -402,9 +606,9 @@ def run_copy(args):
 def run_screen(args):
     """The matrix-core screen on its own: n = 2^20 float32 standard normal
     rows of d = 64 (as float64), 65536 queries, k = 10.  Roofline: the bf16
-    matrix-core peak, with the split contraction's 3 x 2 d bf16 FLOPs per
-    (query, point) pair (qh.ph + ql.ph + qh.pl) as the algorithmic work; the
-    line also gives the 2 d FLOPs per pair of the plain contraction."""
+    matrix-core peak with 2 d FLOPs per (query, point) pair (the plain
+    contraction) as the algorithmic work, over the k_screen kernel's own
+    HIP-event time."""
     import torch
 
     from kdtree import forest as F
@@ -418,210 +622,55 @@ def run_screen(args):
     tree = F.build(data, 32)
     flags = F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_ASSUME_FINITE
     od, oi = F.query_device(tree, q, k, flags=flags)
-    stream = torch.cuda.current_stream()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     for _ in range(args.warmup):
         F.query_device(tree, q, k, od, oi, flags=flags)
     torch.cuda.synchronize()
+    F.timing_enable(True)
+    F.timing_read()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record(stream)
+    for _ in range(args.steps):
         F.query_device(tree, q, k, od, oi, flags=flags)
-        ends[i].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = np.array([s_.elapsed_time(e_) for s_, e_ in zip(starts, ends)])
-    flop = 3 * 2.0 * d * n * nq
-    achieved = flop / (ms.mean() / 1e3) / 1e12
-    peak = 2500.0  # dense bf16 matrix-core peak, TFLOP/s
+    F.timing_enable(False)
+    times = F.timing_read()
+    sc = times["k_screen<32>"]
+    achieved = sc["work"] / (sc["ms"] / 1e3) / 1e12
     m = 64
     otree = OracleTree(data.cpu().numpy(), 32)
-    pd, pi = otree.query(q[:m].cpu().numpy(), k=k, threads=8)
+    pd, pi = otree.query(q[:m].cpu().numpy(), k=k, threads=host_threads())
     parity = bool(np.array_equal(pi, oi[:m].cpu().numpy()) and pd.tobytes() == od[:m].cpu().numpy().tobytes())
     cpu = None
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
+        threads = host_threads()
         qs = q[:256].cpu().numpy()
-        t1 = time.perf_counter()
-        otree.query(qs, k=k, threads=threads)
-        cpu = {"value": qs.shape[0] / (time.perf_counter() - t1), "unit": "queries/s", "cores": threads,
-               "kind": "port", "sample": f"256 of the {nq} queries, {threads} threads (oracle)"}
+        cpu = cpu_timed(lambda: otree.query(qs, k=k, threads=threads), qs.shape[0])
+        cpu.update(unit="queries/s", cores=threads, kind="port", sample=f"256 of the {nq} queries, {threads} threads")
     tree.free()
     print(json.dumps({
         "metric": "k-NN queries/s (exact indices), matrix-core screen, 64-d",
         "value": nq * args.steps / wall, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "dtype": "split bf16 (matrix cores, f32 accumulate) + f64 recheck", "data": "synthetic (torch.randn float32, seed 64)",
+        "dtype": "split bf16 (matrix cores, f32 accumulate) + f64 recheck",
+        "data": "synthetic (torch.randn float32, seed 64)",
         "config": {"workload": "knn-normal-n1048576-d64-q65536-k10-leafsize32-p2"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "what": "3 x 2 d bf16 FLOPs per (query, point) pair over the whole screened query "
-                             "call (matrix-core screen, candidate selection, FP64 recheck, certificate)",
-                     "plain_contraction_tflops": achieved / 3},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_screen<32>",
+                     "kernel_ms_avg": sc["ms"] / sc["launches"],
+                     "what": "2 d FLOPs per (query, point) pair / the k_screen kernel's HIP-event time"},
+        "kernel_times": kernel_summary(times),
         "parity_sample_bit_exact": parity, "cpu_baseline": cpu,
     }), flush=True)
 
 
-def run_c5(args):
-    """Serving: 32 resident trees (n log-uniform in [1e5, 8e6], d alternating
-    16 / 64, float32 standard normal), Poisson requests on uniform trees with
-    batch sizes log-uniform in [1, 65536], k = 10, 200 GB HBM admission budget.
-
-    plain   each request alone, one after another, through kdt_query (host
-            buffers in and out, the FP64 traversal kernel): what a caller of
-            the reference API gets from the plain GPU kernel; and the same
-            requests with their queries resident in HBM, one kdt_query_device
-            call per tree (the plain kernel alone);
-    served  the continuous-batching loop (kdtree.serve.Server): requests are
-            admitted while others run, coalesced into multi-tree launches on
-            several streams, FP32 screens (the matrix-core screen for d = 64,
-            and for d = 16 when it measures faster) with FP64 recheck.
-    Both give identical results; a sample is checked against the oracle and
-    against the plain path.  Latency = arrival to host-visible result."""
-    import torch
-
-    from kdtree import KDTree  # noqa: F401  (loads the library with torch's runtime first)
-    from kdtree import forest as F
-    from kdtree.serve import Request, Server, latency_stats, poisson_requests
-    from kdtree.workloads import C5, c5_queries, c5_specs
-    from oracle import OracleTree
-
-    lib = load_library()
-    specs = c5_specs()
-    dims = [d for _, d in specs]
-    k = C5["k"]
-    smallest = sorted(range(len(specs)), key=lambda t: specs[t][0])[:3]
-    trees, host_data = [], {}
-    t0 = time.perf_counter()
-    for t, (n, d) in enumerate(specs):
-        g = torch.Generator(device="cuda")
-        g.manual_seed(C5["seed"] * 1000 + t)
-        x = torch.randn((n, d), generator=g, device="cuda", dtype=torch.float32).double()
-        trees.append(F.build(x, 32))
-        if t in smallest:
-            host_data[t] = x.cpu().numpy()
-        del x
-    torch.cuda.synchronize()
-    build_s = time.perf_counter() - t0
-
-    # d = 16: matrix-core screen or traversal, whichever is faster on a sample.
-    t16 = [t for t in range(len(specs)) if dims[t] == 16]
-    t16 = sorted(t16, key=lambda t: specs[t][0])[len(t16) // 2]
-    qs = torch.from_numpy(c5_queries(10**6, 4096, 16)).cuda()
-    tune = {}
-    for name, fl in (("traversal_fp32_screen", F.KDT_QUERY_FP32_SCREEN),
-                     ("matrix_screen", F.KDT_QUERY_FP32_SCREEN | 0x8)):
-        F.query_device(trees[t16], qs, k, flags=fl | F.KDT_QUERY_ASSUME_FINITE)
-        tune[name] = _time_steps(lambda: F.query_device(trees[t16], qs, k, flags=fl | F.KDT_QUERY_ASSUME_FINITE),
-                                 3, 1) * 1e3
-    serve_flags = F.KDT_QUERY_FP32_SCREEN | (0x8 if tune["matrix_screen"] < tune["traversal_fp32_screen"] else 0)
-
-    base = poisson_requests(args.requests, rate=1.0, tree_dims=dims, k=k, max_batch=C5["max_batch"],
-                            seed=C5["seed"], make_queries=c5_queries)
-
-    def fresh(rate, subset=None):
-        src = base if subset is None else subset
-        return [Request(r.rid, r.arrival / rate, r.tree, r.queries, r.k) for r in src]
-
-    # Plain: one request at a time, in arrival order, within the time budget;
-    # each request cut to at most 2048 queries (the served comparison on the
-    # same sample runs the same cut requests).
-    plain, plain_t, plain_q = [], 0.0, 0
-    for r0 in base:
-        r = Request(r0.rid, r0.arrival, r0.tree, np.ascontiguousarray(r0.queries[:2048]), r0.k)
-        out_d = np.empty((r.queries.shape[0], k))
-        out_i = np.empty((r.queries.shape[0], k), np.int64)
-        s0 = time.perf_counter()
-        rc = lib.kdt_query(trees[r.tree].handle, ctypes.c_void_p(r.queries.ctypes.data), r.queries.shape[0],
-                           r.queries.shape[1], k, 2.0, float("inf"), 0.0, 0, ctypes.c_void_p(out_d.ctypes.data),
-                           ctypes.c_void_p(out_i.ctypes.data))
-        dt = time.perf_counter() - s0
-        assert rc == 0, lib.kdt_last_error().decode()
-        plain.append((r, dt, out_d, out_i))
-        plain_t += dt
-        plain_q += r.queries.shape[0]
-        if plain_t > args.plain_seconds and len(plain) >= 4:
-            break
-    plain_lat = np.array([dt for _, dt, _, _ in plain]) * 1e3
-    sample = [r for r, _, _, _ in plain]
-
-    # Plain kernel, device-resident: the same sample, each tree's requests
-    # concatenated into one kdt_query_device call (the FP64 traversal, no
-    # screen, queries already in HBM): the plain GPU kernel without the
-    # per-request host copies and calls.
-    by_tree = {}
-    for j, r in enumerate(sample):
-        by_tree.setdefault(r.tree, []).append(j)
-    dev = {t: torch.from_numpy(np.concatenate([sample[j].queries for j in js])).cuda() for t, js in by_tree.items()}
-    for t in dev:
-        F.query_device(trees[t], dev[t][:1], k)
-    torch.cuda.synchronize()
-    s0 = time.perf_counter()
-    pk_out = {t: F.query_device(trees[t], dev[t], k) for t in dev}
-    torch.cuda.synchronize()
-    pk_t = time.perf_counter() - s0
-    pk_exact = all(
-        np.array_equal(pk_out[t][1].cpu().numpy(), np.concatenate([plain[j][3] for j in js]))
-        and pk_out[t][0].cpu().numpy().tobytes() == np.concatenate([plain[j][2] for j in js]).tobytes()
-        for t, js in by_tree.items())
-    del dev, pk_out
-
-    def serve(reqs):
-        server = Server(trees, n_streams=4, hbm_budget_bytes=C5["hbm_budget_bytes"],
-                        max_queries_per_launch=1 << 18, flags=serve_flags)
-        server.run(reqs, time_limit=600)
-        st = latency_stats(reqs)
-        st.update(launches=server.launches, max_inflight=server.max_inflight)
-        return st
-
-    # Served on the plain sample (same work), then the full stream at saturation
-    # (every request already waiting) and at fractions of that capacity.
-    same = fresh(1e12, sample)
-    served_sample = serve(same)
-    exact_vs_plain = all(np.array_equal(s.out_i, pi) and s.out_d.tobytes() == pd.tobytes()
-                         for s, (_, _, pd, pi) in zip(same, plain))
-    sat = serve(fresh(1e12))
-    cap_rps = sat["requests"] / sat["span_s"]
-    loads = {}
-    for frac in (0.5, 0.8):
-        loads[f"{frac:.1f}x_capacity"] = serve(fresh(cap_rps * frac))
-
-    parity = True
-    checked = 0
-    full = fresh(1e12)
-    serve(full)
-    for r in full:
-        if r.tree in host_data and checked < 4:
-            m = min(16, r.queries.shape[0])
-            od, oi = OracleTree(host_data[r.tree], 32).query(r.queries[:m], k=k, threads=8)
-            parity &= bool(np.array_equal(oi, r.out_i[:m]) and od.tobytes() == r.out_d[:m].tobytes())
-            checked += 1
-    out = {
-        "metric": "k-NN queries/s, served (continuous batching + FP32/matrix-core screen) vs plain GPU kernel, "
-                  "exact indices",
-        "value": sat["queries_per_s"], "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f32 screen + f64 recheck", "data": "synthetic (torch.randn float32, seed 5000+t; SplitMix64 "
-                                                     "Box-Muller queries)",
-        "config": {"workload": "serve-32-trees-n1e5..8e6-d{16,64}-k10-batch1..65536-budget200GB",
-                   "n_points": int(sum(n for n, _ in specs)), "requests": args.requests},
-        "plain": {"queries_per_s": plain_q / plain_t, "requests": len(plain), "queries": plain_q,
-                  "note": "first requests in arrival order, each cut to <= 2048 queries",
-                  "p50_ms": float(np.percentile(plain_lat, 50)), "p99_ms": float(np.percentile(plain_lat, 99))},
-        "plain_kernel_device_resident": {"queries_per_s": plain_q / pk_t, "seconds": pk_t,
-                                         "note": "same sample, one kdt_query_device call per tree, FP64 traversal",
-                                         "exact_vs_plain_bitwise": bool(pk_exact)},
-        "served_same_requests": served_sample,
-        "served_vs_plain_same_requests": served_sample["queries_per_s"] / (plain_q / plain_t),
-        "served_vs_plain_kernel_same_requests": served_sample["queries_per_s"] / (plain_q / pk_t),
-        "served_saturation": sat, "served_at_load": loads,
-        "d16_choice_ms_per_4096_queries": tune, "d16_matrix_screen": bool(serve_flags & 0x8),
-        "exact_vs_plain_bitwise": exact_vs_plain, "parity_vs_oracle_sample": parity, "oracle_checked_requests": checked,
-        "build_s": build_s,
-    }
-    for tr in trees:
-        tr.free()
-    print(json.dumps(out), flush=True)
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the workload is a single-GPU serving "
+              "loop and no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+    {"serve": run_serve, "c2": run_c2, "c3": run_c3, "c4": run_c4, "screen": run_screen,
+     "copy": run_copy}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -84,10 +84,15 @@ enum kdt_error {
 
 /* Batch flags (kdt_query_batch_device). */
 #define KDT_BATCH_PER_TREE_LAUNCHES 0x100u /* one launch per item instead of one persistent launch */
-#define KDT_BATCH_NO_ORDER 0x200u          /* keep the given query order (no leaf-order sort) and
-                                              skip the matrix-core screen (items it would take run
-                                              the traversal, same results): the call enqueues
-                                              kernels only, so it can be captured in a graph */
+#define KDT_BATCH_NO_ORDER 0x200u          /* keep the given query order: every item runs its own
+                                              launch sequence on the caller's stream, without
+                                              leaf-order sort; in kdt_query_batch_device also
+                                              without the matrix-core screen (items it would take
+                                              run the traversal, same results), so that the call
+                                              allocates nothing, copies nothing from the host and
+                                              only enqueues kernels (capturable in a graph).  A
+                                              batch plan keeps the screen (its scratch is the
+                                              plan's) */
 
 /* One item of a multi-tree batch: queries (device, n_queries x ndim, ndim =
  * the tree's) against one tree, k neighbours, outputs n_queries x k (device). */
@@ -140,9 +145,10 @@ int kdt_query(const kdt_tree* tree, const double* queries, uint64_t n_queries, u
               double* out_dist, int64_t* out_idx);
 
 /* Same, with queries and outputs in device memory, on `stream`.  The finiteness
- * check of the queries runs on the device and this call synchronises `stream` to
- * report it, unless flags holds KDT_QUERY_ASSUME_FINITE, in which case the call
- * only enqueues the query kernel and returns. */
+ * check of the queries runs on the device first and this call synchronises
+ * `stream` to read it; only finite queries are then queried (the outputs are
+ * untouched on an error) and the call returns once the query is enqueued.  With
+ * KDT_QUERY_ASSUME_FINITE the call skips the check and only enqueues. */
 int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_queries,
                      uint64_t ndim, uint64_t k, double p, double max_distance, double eps,
                      uint32_t flags, double* d_out_dist, int64_t* d_out_idx, void* stream);
@@ -152,9 +158,50 @@ int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_q
  * src/query.rs:29-78); each item's result is identical to kdt_query_device on
  * that item.  The caller guarantees finite queries (as KDT_QUERY_ASSUME_FINITE);
  * p, max_distance and eps apply to every item and are validated like
- * kdt_query.  Runs on `stream` without synchronising. */
+ * kdt_query, and each item like kdt_query_device (dimension, k, buffers,
+ * n_queries * k).  Runs on `stream` without synchronising: it uploads the
+ * item descriptors from host memory and takes stream-ordered scratch, so it is
+ * not capturable in a graph unless flags hold KDT_BATCH_NO_ORDER (use a batch
+ * plan to capture the full path).  Items that run their own launch sequence
+ * may run on side streams of `stream`, forked from it and joined back into it
+ * before the call returns; each caller stream has its own side streams. */
 int kdt_query_batch_device(const kdt_batch_item* items, uint64_t n_items, double p,
                            double max_distance, double eps, uint32_t flags, void* stream);
+
+/* A prepared batch: the items are validated and classified once, one device
+ * region holds the item descriptors and every piece of scratch the batch
+ * needs (leaf-order sort, screen lists), and the plan makes its own side
+ * streams (default priority).  kdt_batch_plan_run then allocates nothing,
+ * copies nothing from the host and only enqueues kernels and event waits on
+ * `stream` (and the plan's side streams, joined back into `stream`), so it
+ * can be captured in a HIP graph and replayed; every run gives the same
+ * results as kdt_query_batch_device on the same items.  The items' trees,
+ * queries and outputs must outlive the plan; runs of one plan must not
+ * overlap (they share its region).  No reference counterpart. */
+typedef struct kdt_batch_plan kdt_batch_plan;
+int kdt_batch_plan_create(const kdt_batch_item* items, uint64_t n_items, double p, double max_distance,
+                          double eps, uint32_t flags, kdt_batch_plan** out);
+int kdt_batch_plan_run(kdt_batch_plan* plan, void* stream);
+uint64_t kdt_batch_plan_bytes(const kdt_batch_plan* plan); /* its device region */
+void kdt_batch_plan_free(kdt_batch_plan* plan);            /* waits for the device first */
+
+/* Kernel timing, a measurement aid with no reference counterpart.  While it is
+ * enabled (kdt_timing_enable(1); returns the previous state), every launch of
+ * the main kernels -- traversal (knn_reg, knn_lds, knn_batch) and matrix-core
+ * screen (k_screen) -- is bracketed by HIP events on the stream it runs on.
+ * kdt_timing_read waits for those events and returns per kernel name the
+ * launches, their summed duration and their summed algorithmic work (FLOPs of
+ * the screen's contraction, 2 ndim per (query, point) pair; queries for a
+ * traversal) since the last read; *count = kernel names (entries beyond
+ * `capacity` are dropped).  Not for use under graph capture. */
+typedef struct kdt_kernel_time {
+  char name[48];
+  uint64_t launches;
+  double ms;
+  double work;
+} kdt_kernel_time;
+int kdt_timing_enable(int on);
+int kdt_timing_read(kdt_kernel_time* out, uint32_t capacity, uint32_t* count);
 
 /* Rows in the caller's original order (n_points x ndim).  Replaces
  * Tree::original_data, src/tree.rs:60-68 (the `data` getter, src/lib.rs:138-146). */

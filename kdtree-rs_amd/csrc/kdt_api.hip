@@ -25,9 +25,48 @@ uint64_t count_tree_nodes(uint64_t n, uint64_t leafsize);
 
 namespace {
 thread_local std::string g_last_error;
-}
+
+// Kernel timing records (kdt_timing_enable / kdt_timing_read).
+struct TimedLaunch {
+  std::string name;
+  double work;
+  hipEvent_t a, b;
+};
+std::atomic<bool> g_timing{false};
+std::mutex g_timing_mu;
+std::vector<TimedLaunch> g_timed;
+}  // namespace
 
 void kdt::set_last_error(std::string msg) { g_last_error = std::move(msg); }
+
+kdt::KernelTimer::KernelTimer(hipStream_t s) : s_(s) {
+  if (!g_timing.load(std::memory_order_relaxed)) return;
+  if (hipEventCreate(&start_) != hipSuccess) {
+    start_ = nullptr;
+    return;
+  }
+  if (hipEventRecord(start_, s_) != hipSuccess) {
+    (void)hipEventDestroy(start_);
+    start_ = nullptr;
+  }
+}
+
+kdt::KernelTimer::~KernelTimer() {
+  if (start_) (void)hipEventDestroy(start_);
+}
+
+void kdt::KernelTimer::stop(const char* name, double work) {
+  if (!start_) return;
+  hipEvent_t b = nullptr;
+  if (hipEventCreate(&b) != hipSuccess) return;
+  if (hipEventRecord(b, s_) != hipSuccess) {
+    (void)hipEventDestroy(b);
+    return;
+  }
+  std::lock_guard<std::mutex> g(g_timing_mu);
+  g_timed.push_back({name, work, start_, b});
+  start_ = nullptr;
+}
 
 namespace {
 
@@ -204,7 +243,6 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
   }
   DevMem<int> flag;
   API_HIP(flag.alloc(1, s));
-  API_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
   DevMem<double> dq, dd;
   DevMem<int64_t> di;
   if (!q_on_device) API_HIP(dq.alloc(tile * ndim, s));
@@ -212,6 +250,10 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
     API_HIP(dd.alloc(tile * k, s));
     API_HIP(di.alloc(tile * k, s));
   }
+  // Each tile: queries in, all_finite (src/kernel.rs:67-74, checked before
+  // eps, max_distance and p, src/query.rs:46-62), and only when the tile is
+  // finite the query and the results out -- a call that fails leaves the
+  // caller's output buffers untouched by this tile.
   for (uint64_t b = 0; b < nq; b += tile) {
     const uint64_t m = std::min(tile, nq - b);
     const double* qsrc = q + b * ndim;
@@ -220,13 +262,13 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
       API_HIP(staged_copy({{dq.p, qsrc, m * ndim * 8}}, false, s));
       dqp = dq.p;
     }
-    // all_finite, src/kernel.rs:67-74 (checked before eps, max_distance, p)
+    API_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
     kdt::launch_check_finite(dqp, m * ndim, flag.p, s);
+    int h = 0;
+    API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    API_HIP(hipStreamSynchronize(s));
+    if (h) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
     if (scalar_rc) {
-      int h = 0;
-      API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-      API_HIP(hipStreamSynchronize(s));
-      if (h) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
       if (b + m >= nq) return fail(scalar_rc, smsg);
       continue;
     }
@@ -241,13 +283,11 @@ int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t
     std::string msg;
     const int rc = kdt::launch_query(t, qpt, dqp, od, oi, s, msg);
     if (rc) return fail(rc, msg);
-    if (!out_on_device)
+    if (!out_on_device) {
       API_HIP(staged_copy({{out_d + b * k, dd.p, m * k * 8}, {out_i + b * k, di.p, m * k * 8}}, true, s));
+      API_HIP(hipStreamSynchronize(s));
+    }
   }
-  int h = 0;
-  API_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  API_HIP(hipStreamSynchronize(s));
-  if (h) return fail(KDT_ERR_NON_FINITE, "all coordinates must be finite");
   return KDT_OK;
 }
 
@@ -389,5 +429,43 @@ void kdt_free(kdt_tree* tree) {
 }
 
 const char* kdt_last_error(void) { return g_last_error.c_str(); }
+
+int kdt_timing_enable(int on) { return g_timing.exchange(on != 0) ? 1 : 0; }
+
+int kdt_timing_read(kdt_kernel_time* out, uint32_t capacity, uint32_t* count) {
+  if (!count || (!out && capacity)) return fail(KDT_ERR_ARGUMENT, "null argument");
+  std::vector<TimedLaunch> recs;
+  {
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    recs.swap(g_timed);
+  }
+  std::vector<kdt_kernel_time> agg;
+  hipError_t err = hipSuccess;
+  for (TimedLaunch& r : recs) {
+    float ms = 0.0f;
+    hipError_t e = hipEventSynchronize(r.b);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+    if (e != hipSuccess) {
+      err = e;
+      continue;
+    }
+    size_t j = 0;
+    while (j < agg.size() && std::strncmp(agg[j].name, r.name.c_str(), sizeof agg[j].name) != 0) ++j;
+    if (j == agg.size()) {
+      kdt_kernel_time z{};
+      std::strncpy(z.name, r.name.c_str(), sizeof z.name - 1);
+      agg.push_back(z);
+    }
+    agg[j].launches += 1;
+    agg[j].ms += ms;
+    agg[j].work += r.work;
+  }
+  for (size_t j = 0; j < agg.size() && j < capacity; ++j) out[j] = agg[j];
+  *count = (uint32_t)agg.size();
+  if (err != hipSuccess) return fail(KDT_ERR_DEVICE, std::string("kernel timing: ") + hipGetErrorString(err));
+  return KDT_OK;
+}
 
 }  // extern "C"

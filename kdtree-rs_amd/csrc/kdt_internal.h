@@ -19,8 +19,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace kdt {
 
@@ -56,6 +58,94 @@ struct Tree {
   double pmax = 0.0;     // upper bound on max |row32|
 };
 
+// Device scratch of one call (leaf-order keys, screen lists, batch
+// descriptors).  Two modes:
+//   stream    hipMallocAsync on the call's stream; release() (or the
+//             destructor) frees with hipFreeAsync on the same stream, ordered
+//             after the work enqueued so far;
+//   region    bump allocation inside a caller-given device region: nothing is
+//             allocated or freed, so the call can be captured in a HIP graph
+//             or made by a caller that cannot allocate.  A request that does
+//             not fit fails with hipErrorOutOfMemory; no kernel is launched
+//             on a short region.
+// Every piece is 256-byte aligned.  SCRATCH_ALIGN pads each piece so that
+// size estimates can add `scratch_piece(bytes)` per piece.
+constexpr size_t SCRATCH_ALIGN = 256;
+inline size_t scratch_piece(size_t bytes) {
+  return (std::max<size_t>(bytes, 1) + SCRATCH_ALIGN - 1) & ~(SCRATCH_ALIGN - 1);
+}
+
+class Scratch {
+ public:
+  explicit Scratch(hipStream_t s) : s_(s) {}
+  Scratch(void* base, size_t bytes, hipStream_t s) : s_(s), base_((char*)base), cap_(bytes), region_(true) {}
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  ~Scratch() { release(); }
+
+  template <typename T>
+  hipError_t get(T** p, size_t count) {
+    const size_t bytes = scratch_piece(count * sizeof(T));
+    *p = nullptr;
+    if (region_) {
+      if (bytes > cap_ - used_) return hipErrorOutOfMemory;  // used_ <= cap_ always
+      *p = reinterpret_cast<T*>(base_ + used_);
+      used_ += bytes;
+      return hipSuccess;
+    }
+    void* q = nullptr;
+    const hipError_t e = hipMallocAsync(&q, bytes, s_);
+    if (e == hipSuccess) {
+      owned_.push_back(q);
+      *p = reinterpret_cast<T*>(q);
+    }
+    return e;
+  }
+  void release() {
+    for (void* q : owned_) (void)hipFreeAsync(q, s_);
+    owned_.clear();
+  }
+  bool region() const { return region_; }
+  hipStream_t stream() const { return s_; }
+  size_t used() const { return used_; }
+
+ private:
+  hipStream_t s_;
+  char* base_ = nullptr;
+  size_t cap_ = 0, used_ = 0;
+  bool region_ = false;
+  std::vector<void*> owned_;
+};
+
+// Exclusive prefix sum of n uint32 counts into `first` (a separate array), by
+// three plain kernels (block scans, a scan of the block sums, the add): no
+// memset and no library scratch, so it records cleanly into a HIP graph.
+// Scratch: scan_scratch_bytes(n).
+size_t scan_scratch_bytes(uint64_t n);
+hipError_t exclusive_scan_u32(const uint32_t* count, uint32_t* first, uint64_t n, Scratch& sc, hipStream_t s);
+// Counting-sort pieces shared by the single-tree and batch leaf orders:
+// clear n counts; order[first[key[i]] + rank[i]] = i for i < nq.
+void launch_zero_u32(uint32_t* p, uint64_t n, hipStream_t s);
+void launch_order_scatter(const uint32_t* key, const uint32_t* rank, const uint32_t* first, uint64_t nq,
+                          uint32_t* order, hipStream_t s);
+
+// Kernel timing (kdt_timing_enable): while it is on, a KernelTimer brackets
+// one launch with HIP events on the launch's stream and files the pair under
+// `name` with the launch's algorithmic work (FLOPs for the matrix-core
+// screen, queries for a traversal) for kdt_timing_read.  Off, it does nothing.
+class KernelTimer {
+ public:
+  explicit KernelTimer(hipStream_t s);
+  ~KernelTimer();
+  void stop(const char* name, double work);
+  KernelTimer(const KernelTimer&) = delete;
+  KernelTimer& operator=(const KernelTimer&) = delete;
+
+ private:
+  hipStream_t s_;
+  hipEvent_t start_ = nullptr;
+};
+
 // Build on the device from row-major f64 data already in device memory.
 // Returns 0 or a kdt_error code; `msg` receives the reason.
 int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg);
@@ -69,8 +159,8 @@ struct QueryParams {
   bool matrix_screen = false;  // matrix-core screen also for 8 <= ndim < 32
   int group = 0;               // lanes per query requested (8, 16, 32, 64), 0 = from k
   bool allow_order = true;    // large batches may run in leaf order
-  bool allow_scratch = true;  // stream-ordered scratch allowed (the matrix-core screen needs it);
-                              // false for a pure enqueue, e.g. a launch sequence under graph capture
+  bool allow_scratch = true;  // scratch allowed, stream-ordered or from a region (the matrix-core
+                              // screen needs it); false for a pure enqueue with no region
 };
 
 // Lanes per query requested by KDT_QUERY_GROUP bits of the query flags (0 = none).
@@ -101,14 +191,26 @@ std::string rust_f64(double x);
 // Thread-local message behind kdt_last_error().
 void set_last_error(std::string msg);
 
-// Launch the exact k-NN query; d_q [nq x d] f64, outputs [nq x k].
+// Launch the exact k-NN query; d_q [nq x d] f64, outputs [nq x k].  Scratch
+// comes from `region` when given (no allocation), else from stream-ordered
+// allocations on `s`.
 int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
-                 int64_t* d_idx, hipStream_t s, std::string& msg);
+                 int64_t* d_idx, hipStream_t s, std::string& msg, Scratch* region = nullptr);
+// Upper bound of the scratch bytes launch_query takes for (t, qp).
+size_t query_scratch_bytes(const Tree& t, const QueryParams& qp);
+
+// Leaf order of a query batch (kdt_order.hip): d_order[nq] receives the
+// queries grouped by the leaf their near path reaches.
+int order_queries(const Tree& t, const double* d_q, uint64_t nq, uint32_t* d_order, Scratch& sc,
+                  hipStream_t s, std::string& msg);
+size_t order_scratch_bytes(const Tree& t, uint64_t nq);
 
 // Matrix-core screen for 8 <= ndim <= 64 (kdt_screen.hip).
 struct KArgs;
 bool screen_applies(const Tree& t, const QueryParams& qp);
-int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg);
+int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
+                  std::string& msg);
+size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp);
 
 // Non-finite check of n doubles; sets *d_flag to 1 if any is NaN or infinite.
 void launch_check_finite(const double* d_x, uint64_t n, int* d_flag, hipStream_t s);

@@ -14,9 +14,6 @@
 
 namespace kdt {
 
-int order_queries(const Tree& t, const double* d_q, uint64_t nq, uint32_t* d_order, hipStream_t s,
-                  std::string& msg);
-
 namespace {
 constexpr size_t LDS_LIST_LIMIT = 60000;  // bytes of dynamic LDS per block
 constexpr uint64_t ORDER_MIN_QUERIES = 8192; // smaller batches run in the given order
@@ -113,26 +110,36 @@ int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double*
   return 0;
 }
 
+bool order_applies(const QueryParams& qp) {
+  return qp.allow_order && qp.nq >= ORDER_MIN_QUERIES && qp.nq <= 0xFFFFFFFFull;
+}
+
+size_t query_scratch_bytes(const Tree& t, const QueryParams& qp) {
+  if (qp.nq == 0) return 0;
+  if (screen_applies(t, qp)) return screen_scratch_bytes(t, qp);
+  if (order_applies(qp)) return scratch_piece(qp.nq * 4) + order_scratch_bytes(t, qp.nq);
+  return 0;
+}
+
 int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double* d_out,
-                 int64_t* d_idx, hipStream_t s, std::string& msg) {
+                 int64_t* d_idx, hipStream_t s, std::string& msg, Scratch* region) {
   KArgs a;
   build_kargs(t, qp, d_q, d_out, d_idx, a);
   if (qp.nq == 0) return 0;
-  if (screen_applies(t, qp)) return launch_screen(t, qp, a, s, msg);
+  Scratch local(s);  // stream-ordered scratch when no region is given
+  Scratch& sc = region ? *region : local;
+  if (screen_applies(t, qp)) return launch_screen(t, qp, a, sc, s, msg);
 
   // Large batches run grouped by leaf (see kdt_order.hip).
-  uint32_t* order = nullptr;
-  if (qp.allow_order && qp.nq >= ORDER_MIN_QUERIES && qp.nq <= 0xFFFFFFFFull) {
-    hipError_t e = hipMallocAsync((void**)&order, qp.nq * 4, s);
+  if (order_applies(qp)) {
+    uint32_t* order = nullptr;
+    const hipError_t e = sc.get(&order, qp.nq);
     if (e != hipSuccess) {
       msg = std::string("query ordering: ") + hipGetErrorString(e);
       return 100;
     }
-    const int rc = order_queries(t, d_q, qp.nq, order, s, msg);
-    if (rc) {
-      (void)hipFreeAsync(order, s);
-      return rc;
-    }
+    const int rc = order_queries(t, d_q, qp.nq, order, sc, s, msg);
+    if (rc) return rc;
     a.order = order;
   }
 
@@ -154,7 +161,6 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
     }
     rc = launch_knn_lds(a, qp.metric, s);
   }
-  if (order) (void)hipFreeAsync(order, s);
   if (rc != 0) {
     msg = "no kernel for this query configuration";
     return 101;

@@ -41,6 +41,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <type_traits>
 
@@ -800,8 +801,12 @@ inline void launch_reg(const KArgs& a, hipStream_t s) {
   // One group per query (measured faster than a resident grid that hands
   // each group several queries); a larger batch loops (knn_body's next()).
   const uint64_t blocks = std::min<uint64_t>((a.nq + GPB - 1) / GPB, 1u << 30);
+  KernelTimer tm(s);
   if (a.stk <= 32) knn_reg_kernel<DT, G, M, SCREEN, 32><<<(uint32_t)blocks, 256, 0, s>>>(a);
   else knn_reg_kernel<DT, G, M, SCREEN, STK_MAX><<<(uint32_t)blocks, 256, 0, s>>>(a);
+  char name[48];
+  std::snprintf(name, sizeof name, "knn_reg<%d,%d,%d%s>", DT, G, M, SCREEN ? ",fp32" : "");
+  tm.stop(name, (double)a.nq);
 }
 
 // Widths with a compiled kernel: the reference's own monomorphised list

@@ -12,13 +12,18 @@ int launch_knn_lds(const KArgs& a, int metric, hipStream_t s) {
   // nothing to do; 16384 groups still give each flagged query its own group in
   // all but extreme cases.
   if (a.only && blocks > (1u << 14)) blocks = 1u << 14;
+  KernelTimer tm(s);
   switch (metric) {
-    case M_L1: knn_lds_kernel<M_L1><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
-    case M_L2: knn_lds_kernel<M_L2><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
-    case M_LINF: knn_lds_kernel<M_LINF><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
-    case M_LP: knn_lds_kernel<M_LP><<<(uint32_t)blocks, 64, shm, s>>>(a); return 0;
+    case M_L1: knn_lds_kernel<M_L1><<<(uint32_t)blocks, 64, shm, s>>>(a); break;
+    case M_L2: knn_lds_kernel<M_L2><<<(uint32_t)blocks, 64, shm, s>>>(a); break;
+    case M_LINF: knn_lds_kernel<M_LINF><<<(uint32_t)blocks, 64, shm, s>>>(a); break;
+    case M_LP: knn_lds_kernel<M_LP><<<(uint32_t)blocks, 64, shm, s>>>(a); break;
+    default: return -1;
   }
-  return -1;
+  char name[48];
+  std::snprintf(name, sizeof name, "knn_lds<%d%s>", metric, a.only ? ",flagged" : "");
+  tm.stop(name, (double)a.nq);
+  return 0;
 }
 
 }  // namespace kdt

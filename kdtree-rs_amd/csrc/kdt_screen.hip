@@ -686,9 +686,19 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   }
 }
 
-template <typename T>
-int dev_alloc(T** p, size_t count, hipStream_t s) {
-  return hipMallocAsync((void**)p, std::max<size_t>(1, count) * sizeof(T), s) == hipSuccess ? 0 : 1;
+// Point slices per query tile: enough waves to fill the device, at least 8
+// tiles per slice.  For k > KP, at least k / 4 interleaved slices: a query's
+// top k then spread over at least k / 2 lists of KP, about two per list, so a
+// list rarely fills below tau_A (the completeness check flags the query when
+// one does).  Returns the slice count; *per = tiles per slice.
+uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
+  const uint64_t qtiles = (qp.nq + 31) / 32;
+  const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
+  uint64_t slices = std::max<uint64_t>(1, (4096 + qtiles - 1) / qtiles);
+  if (qp.k > (uint64_t)KP) slices = std::max<uint64_t>(slices, (qp.k + 3) / 4);
+  slices = std::min<uint64_t>({slices, MAX_SLICES, std::max<uint32_t>(1, n_tiles / 8)});
+  *per = (uint32_t)((n_tiles + slices - 1) / slices);
+  return (n_tiles + *per - 1) / *per;
 }
 
 }  // namespace
@@ -699,38 +709,36 @@ bool screen_applies(const Tree& t, const QueryParams& qp) {
          qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 8)) && t.d <= 64;
 }
 
-int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipStream_t s, std::string& msg) {
+size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp) {
+  uint32_t per = 0;
+  const uint64_t slices = screen_slices(t, qp, &per);
+  const uint64_t nq = qp.nq;
+  return 2 * scratch_piece(nq * 2 * slices * KP * 4) + scratch_piece(nq * 4) + scratch_piece(nq * CAND_CAP * 4) +
+         scratch_piece(nq);
+}
+
+int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
+                  std::string& msg) {
   const uint64_t nq = qp.nq;
   const uint32_t k = (uint32_t)qp.k;
   const uint64_t qtiles = (nq + 31) / 32;
   const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
-  // Slices of the points per query tile: enough waves to fill the device,
-  // at least 8 tiles per slice.
-  // For k > KP, at least k / 4 interleaved slices: a query's top k then spread over at
-  // least k / 2 lists of KP, about two per list, so a list rarely fills below
-  // tau_A (the completeness check flags the query when one does).
-  uint64_t slices = std::max<uint64_t>(1, (4096 + qtiles - 1) / qtiles);
-  if (k > (uint32_t)KP) slices = std::max<uint64_t>(slices, (k + 3) / 4);
-  slices = std::min<uint64_t>({slices, MAX_SLICES, std::max<uint32_t>(1, n_tiles / 8)});
-  const uint32_t per = (uint32_t)((n_tiles + slices - 1) / slices);
-  slices = (n_tiles + per - 1) / per;
+  uint32_t per = 0;
+  const uint64_t slices = screen_slices(t, qp, &per);
 
+  // Scratch: every piece is written in full before it is read (the lists by
+  // k_screen, counts, candidates and flags by k_screen_select), so nothing
+  // needs clearing.
   float* part_a = nullptr;
   uint32_t *part_i = nullptr, *cnt = nullptr, *cand = nullptr;
   uint8_t* flag = nullptr;
-  int bad = 0;
-  bad |= dev_alloc(&part_a, nq * 2 * slices * KP, s);
-  bad |= dev_alloc(&part_i, nq * 2 * slices * KP, s);
-  bad |= dev_alloc(&cnt, nq, s);
-  bad |= dev_alloc(&cand, nq * CAND_CAP, s);
-  bad |= dev_alloc(&flag, nq, s);
-  auto release = [&]() {
-    for (void* p : {(void*)part_a, (void*)part_i, (void*)cnt, (void*)cand, (void*)flag})
-      if (p) (void)hipFreeAsync(p, s);
-  };
-  if (bad) {
-    release();
-    msg = "screen scratch allocation failed";
+  hipError_t e = sc.get(&part_a, nq * 2 * slices * KP);
+  if (e == hipSuccess) e = sc.get(&part_i, nq * 2 * slices * KP);
+  if (e == hipSuccess) e = sc.get(&cnt, nq);
+  if (e == hipSuccess) e = sc.get(&cand, nq * CAND_CAP);
+  if (e == hipSuccess) e = sc.get(&flag, nq);
+  if (e != hipSuccess) {
+    msg = std::string("screen scratch: ") + hipGetErrorString(e);
     return 100;
   }
   SArgs sa{};
@@ -749,9 +757,15 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;
   const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
   const dim3 grid((uint32_t)((qtiles + sw - 1) / sw), (uint32_t)slices);
-  if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(sa);
-  else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(sa);
-  else k_screen<32><<<grid, 64 * sw, 0, s>>>(sa);
+  {
+    // Algorithmic work: 2 D FLOPs per (query, point) pair of the contraction.
+    KernelTimer tm(s);
+    if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(sa);
+    else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(sa);
+    else k_screen<32><<<grid, 64 * sw, 0, s>>>(sa);
+    tm.stop(ks == 8 ? "k_screen<8>" : ks == 16 ? "k_screen<16>" : "k_screen<32>",
+            2.0 * (double)t.d * (double)t.n * (double)nq);
+  }
 
   SelArgs se{};
   se.part_a = part_a;
@@ -793,12 +807,11 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, hipSt
   fb.order = nullptr;
   fb.only = flag;
   const int rc = qp.screen_only ? 0 : launch_knn_lds(fb, M_L2, s);
-  release();
   if (rc) {
     msg = "no fallback kernel";
     return 101;
   }
-  const hipError_t e = hipGetLastError();
+  e = hipGetLastError();
   if (e != hipSuccess) {
     msg = std::string("screen launch: ") + hipGetErrorString(e);
     return 100;

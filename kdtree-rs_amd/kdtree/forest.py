@@ -6,11 +6,15 @@ single-tree query of the same tree.  Device memory and streams come from
 PyTorch; the work is kdt_build_device / kdt_query_device /
 kdt_query_batch_device of libkdtree_hip.so.
 
-Three ways to run a batch of (tree, queries, k) items:
-  persistent  one kdt_query_batch_device launch: groups of lanes take queries
-              of every tree from one counter (default)
-  per_tree    one launch per item (the baseline)
-  graph       the per-tree launches captured once in a HIP graph and replayed
+Ways to run a batch of (tree, queries, k) items:
+  persistent  one kdt_query_batch_device call: groups of lanes take queries
+              of every tree from one counter per class (default)
+  per_tree    one launch sequence per item (the baseline)
+  plan        a batch plan (kdt_batch_plan_create once, kdt_batch_plan_run per
+              call): the persistent path with descriptors and scratch prepared
+  graph       the plan's run captured once in a HIP graph and replayed: the
+              same launches (leaf ordering, shared persistent launches, the
+              matrix-core screen) as one graph launch
 """
 
 from __future__ import annotations
@@ -52,14 +56,51 @@ def _torch():
     return torch
 
 
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64), ("ms", ctypes.c_double),
+                ("work", ctypes.c_double)]
+
+
 def _lib():
     lib = load_library()
     if not getattr(lib, "_batch_bound", False):
+        vp = ctypes.c_void_p
         lib.kdt_query_batch_device.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_uint64, ctypes.c_double,
-                                               ctypes.c_double, ctypes.c_double, ctypes.c_uint32, ctypes.c_void_p]
+                                               ctypes.c_double, ctypes.c_double, ctypes.c_uint32, vp]
         lib.kdt_query_batch_device.restype = ctypes.c_int
+        lib.kdt_batch_plan_create.argtypes = [ctypes.POINTER(BatchItem), ctypes.c_uint64, ctypes.c_double,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_uint32,
+                                              ctypes.POINTER(vp)]
+        lib.kdt_batch_plan_create.restype = ctypes.c_int
+        lib.kdt_batch_plan_run.argtypes = [vp, vp]
+        lib.kdt_batch_plan_run.restype = ctypes.c_int
+        lib.kdt_batch_plan_bytes.argtypes = [vp]
+        lib.kdt_batch_plan_bytes.restype = ctypes.c_uint64
+        lib.kdt_batch_plan_free.argtypes = [vp]
+        lib.kdt_batch_plan_free.restype = None
+        lib.kdt_timing_enable.argtypes = [ctypes.c_int]
+        lib.kdt_timing_enable.restype = ctypes.c_int
+        lib.kdt_timing_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.kdt_timing_read.restype = ctypes.c_int
         lib._batch_bound = True
     return lib
+
+
+def timing_enable(on: bool = True) -> bool:
+    """Turn the library's kernel timing on or off; returns the previous state."""
+    return bool(_lib().kdt_timing_enable(1 if on else 0))
+
+
+def timing_read() -> dict:
+    """{kernel name: {"launches", "ms", "work"}} for the launches timed since the
+    last read (waits for their events)."""
+    lib = _lib()
+    cap = 256
+    buf = (KernelTime * cap)()
+    n = ctypes.c_uint32()
+    _check(lib, lib.kdt_timing_read(buf, cap, ctypes.byref(n)))
+    return {buf[j].name.decode(): {"launches": int(buf[j].launches), "ms": float(buf[j].ms),
+                                   "work": float(buf[j].work)} for j in range(min(n.value, cap))}
 
 
 def _check(lib, rc):
@@ -87,6 +128,20 @@ class DeviceTree:
         if self.handle is not None and self.handle.value:
             _lib().kdt_free(self.handle)
             self.handle = None
+
+
+def export(tree: DeviceTree):
+    """(rows in tree order, original indices, node records) of a built tree."""
+    from ._core import NODE_DTYPE
+
+    lib = _lib()
+    nn = int(lib.kdt_n_nodes(tree.handle))
+    rows = np.empty((tree.n, tree.d), np.float64)
+    idx = np.empty(tree.n, np.uint32)
+    nodes = np.empty(nn, NODE_DTYPE)
+    _check(lib, lib.kdt_export(tree.handle, ctypes.c_void_p(rows.ctypes.data), ctypes.c_void_p(idx.ctypes.data),
+                               ctypes.c_void_p(nodes.ctypes.data), None))
+    return rows, idx, nodes
 
 
 def build(data, leafsize: int, stream=None) -> DeviceTree:
@@ -197,39 +252,69 @@ class Batch:
             self.items.append((tree, q, k, out_d, out_i))
             self.arr[j] = BatchItem(tree.handle.value, q.data_ptr(), q.shape[0], tree.d, k, out_d.data_ptr(),
                                     out_i.data_ptr(), tree.group, tree.flags)
-        self.graph = None
+        self.graphs = {}
+        self.plans = {}
 
     @property
     def n_queries(self) -> int:
         return sum(q.shape[0] for _, q, _, _, _ in self.items)
 
+    def plan(self, p: float = 2.0, extra_flags: int = 0) -> ctypes.c_void_p:
+        """The batch plan for (p, flags), made on first use."""
+        key = (p, extra_flags)
+        if key not in self.plans:
+            lib = _lib()
+            h = ctypes.c_void_p()
+            _check(lib, lib.kdt_batch_plan_create(self.arr, len(self.items), p, float("inf"), 0.0, extra_flags,
+                                                  ctypes.byref(h)))
+            self.plans[key] = h
+        return self.plans[key]
+
     def run(self, mode: str = "persistent", stream=None, p: float = 2.0, extra_flags: int = 0):
         lib = _lib()
         if mode == "graph":
-            if self.graph is None:
-                self._capture(p, extra_flags)
-            self.graph.replay()
+            key = (p, extra_flags)
+            if key not in self.graphs:
+                self.graphs[key] = self._capture(p, extra_flags)
+            self.graphs[key].replay()
+            return
+        if mode == "plan":
+            _check(lib, lib.kdt_batch_plan_run(self.plan(p, extra_flags), _stream_ptr(stream)))
             return
         flags = extra_flags | (KDT_BATCH_PER_TREE_LAUNCHES if mode == "per_tree" else 0)
         _check(lib, lib.kdt_query_batch_device(self.arr, len(self.items), p, float("inf"), 0.0, flags,
                                                _stream_ptr(stream)))
 
     def _capture(self, p, extra_flags):
+        """Capture one run of the batch plan (a pure enqueue) in a HIP graph."""
         torch = _torch()
         lib = _lib()
-        flags = extra_flags | KDT_BATCH_PER_TREE_LAUNCHES | KDT_BATCH_NO_ORDER
+        plan = self.plan(p, extra_flags)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm the kernels outside capture
-            _check(lib, lib.kdt_query_batch_device(self.arr, len(self.items), p, float("inf"), 0.0, flags,
-                                                   _stream_ptr(s)))
+            _check(lib, lib.kdt_batch_plan_run(plan, _stream_ptr(s)))
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            _check(lib, lib.kdt_query_batch_device(self.arr, len(self.items), p, float("inf"), 0.0, flags,
-                                                   _stream_ptr(None)))
-        self.graph = g
+            _check(lib, lib.kdt_batch_plan_run(plan, _stream_ptr(None)))
+        return g
+
+    def close(self):
+        """Release the graphs and plans (waits for the device)."""
+        self.graphs.clear()
+        lib = _lib()
+        for h in self.plans.values():
+            lib.kdt_batch_plan_free(h)
+        self.plans.clear()
+
+    def __del__(self):
+        try:
+            if self.plans:
+                self.close()
+        except Exception:
+            pass
 
     def results(self, j: int):
         _, _, _, out_d, out_i = self.items[j]

@@ -15,13 +15,20 @@ time.  The dispatcher loop:
   * the first stream is a lane for small requests (at most `small_max`
     queries): when it is free it takes every pending small request, so that a
     small request never waits behind launches of large ones; the other
-    streams take every request in arrival order.
+    streams take every request in arrival order;
+  * a request of more than `slice_max` queries is admitted as slices of at
+    most `slice_max` rows, and a launch takes at most one slice of each
+    request: a large request then spreads over several launches (and
+    streams) instead of holding one stream for its whole length, so the
+    longest launch, and with it the wait of a request that arrives behind
+    it, stays bounded.  The request completes with its last slice.
 
 Requests that arrive while earlier launches run are admitted at the next
 dispatch and run concurrently on another stream, so work from different
 requests overlaps on the device.  Every request's rows are computed by the
 same per-query descent whatever else is in the launch, so a result is the
-same alone and in any mix.  Latency = host-visible completion - arrival.
+same alone, sliced and in any mix.  Latency = host-visible completion -
+arrival.
 """
 
 from __future__ import annotations
@@ -49,10 +56,33 @@ class Request:
 
 
 @dataclass
+class Slice:
+    """Rows [lo, hi) of a request: the unit a launch takes."""
+
+    req: Request
+    lo: int
+    hi: int
+
+    @property
+    def queries(self) -> np.ndarray:
+        return self.req.queries[self.lo:self.hi]
+
+    @property
+    def rid(self) -> int:
+        return self.req.rid
+
+
+def split_request(r: Request, slice_max: int) -> list:
+    """The slices of `r`, at most `slice_max` rows each (one slice if it fits)."""
+    m = r.queries.shape[0]
+    return [Slice(r, lo, min(m, lo + slice_max)) for lo in range(0, m, slice_max)]
+
+
+@dataclass
 class _Launch:
     stream: object
     event: object
-    groups: list  # (requests, pinned host distances, pinned host indices), one per item
+    groups: list  # (slices, pinned host distances, pinned host indices), one per item
     dev: list = field(default_factory=list)
     nbytes: int = 0
 
@@ -60,7 +90,7 @@ class _Launch:
 class Server:
     def __init__(self, trees, n_streams: int = 4, hbm_budget_bytes: float = 200e9,
                  max_queries_per_launch: int = 1 << 18, p: float = 2.0, flags: int = 0,
-                 small_max: int = 4096):
+                 small_max: int = 4096, slice_max: int = 8192):
         import torch
 
         self.torch = torch
@@ -71,32 +101,34 @@ class Server:
         self.p = p
         self.flags = flags
         self.small_max = small_max
+        self.slice_max = slice_max
         self.lib = F._lib()
         self.max_inflight = 0
         self.launches = 0
 
-    def _bytes(self, r: Request) -> int:
-        """Device bytes a request holds while in flight: queries, outputs and
+    def _bytes(self, sl) -> int:
+        """Device bytes a slice holds while in flight: queries, outputs and
         an upper bound of the matrix-core screen's scratch (two 16-entry lists
         of 8-byte entries per query and point slice, with the launch's slice
         rule: enough slices for 4096 query-tile waves, at least k / 4 for
         k > 16, at most 512; then 64 candidates, a count and a flag per query)."""
-        m, d = r.queries.shape
+        m, d = sl.queries.shape
+        k = sl.req.k
         qtiles = -(-m // 32)
-        slices = max(-(-4096 // qtiles), -(-r.k // 4) if r.k > 16 else 1)
+        slices = max(-(-4096 // qtiles), -(-k // 4) if k > 16 else 1)
         slices = min(slices, 512)
-        return m * d * 8 + m * r.k * 16 + m * (256 * slices + 261)
+        return m * d * 8 + m * k * 16 + m * (256 * slices + 261)
 
-    def _dispatch(self, reqs, stream) -> _Launch:
+    def _dispatch(self, slices, stream) -> _Launch:
         torch = self.torch
         by_item: dict = {}
-        for r in reqs:  # requests on one tree with one k share an item
-            by_item.setdefault((r.tree, r.k), []).append(r)
+        for sl in slices:  # slices on one tree with one k share an item
+            by_item.setdefault((sl.req.tree, sl.req.k), []).append(sl)
         items = (F.BatchItem * len(by_item))()
         dev, groups = [], []
         with torch.cuda.stream(stream):
             for j, ((tree_id, k), rs) in enumerate(by_item.items()):
-                q = rs[0].queries if len(rs) == 1 else np.concatenate([r.queries for r in rs])
+                q = np.ascontiguousarray(rs[0].queries) if len(rs) == 1 else np.concatenate([x.queries for x in rs])
                 m = q.shape[0]
                 qh = torch.from_numpy(q).pin_memory()
                 qd = qh.to("cuda", non_blocking=True)
@@ -117,7 +149,7 @@ class Server:
                 groups.append((rs, hd, hi))
             ev = torch.cuda.Event()
             ev.record(stream)
-        return _Launch(stream, ev, groups, dev, sum(self._bytes(r) for r in reqs))
+        return _Launch(stream, ev, groups, dev, sum(self._bytes(x) for x in slices))
 
     def run(self, requests, time_limit: float = 600.0):
         """Serve `requests` (sorted by arrival); returns them with results and completion times."""
@@ -125,6 +157,7 @@ class Server:
         upcoming = deque(sorted(requests, key=lambda r: r.arrival))
         inflight: list[_Launch] = []
         free = list(self.streams)
+        left: dict = {}  # request id -> slices not yet retired
         used = 0
         t0 = time.perf_counter()
         while upcoming or pending or inflight:
@@ -132,17 +165,27 @@ class Server:
             if now > time_limit:
                 raise TimeoutError("serving run exceeded its time limit")
             while upcoming and upcoming[0].arrival <= now:
-                pending.append(upcoming.popleft())
+                r = upcoming.popleft()
+                m = r.queries.shape[0]
+                r.out_d = np.empty((m, r.k), np.float64)
+                r.out_i = np.empty((m, r.k), np.int64)
+                parts = split_request(r, self.slice_max)
+                left[id(r)] = len(parts)
+                pending.extend(parts)
             for L in list(inflight):
                 if L.event.query():
                     tdone = time.perf_counter() - t0
-                    for rs, hd, hi in L.groups:
+                    for sls, hd, hi in L.groups:
                         d_all, i_all = hd.numpy(), hi.numpy()
                         off = 0
-                        for r in rs:
-                            m = r.queries.shape[0]
-                            r.out_d, r.out_i, r.done = d_all[off:off + m], i_all[off:off + m], tdone
+                        for sl in sls:
+                            m = sl.hi - sl.lo
+                            sl.req.out_d[sl.lo:sl.hi] = d_all[off:off + m]
+                            sl.req.out_i[sl.lo:sl.hi] = i_all[off:off + m]
                             off += m
+                            left[id(sl.req)] -= 1
+                            if left[id(sl.req)] == 0:
+                                sl.req.done = tdone
                     inflight.remove(L)
                     free.append(L.stream)
                     used -= L.nbytes
@@ -165,31 +208,34 @@ class Server:
 
 
 def plan_launch(pending, free, lane, small_max, cap, budget_left, busy, nbytes):
-    """The next launch: (requests, stream, requests still pending).
+    """The next launch: (units, stream, units still pending).
 
-    `pending` holds requests in arrival order, `free` the idle streams, `lane`
-    the stream kept for small requests (at most `small_max` queries).  When the
-    lane is idle and a small request waits, the launch takes pending small
-    requests only, on the lane; otherwise it takes requests in arrival order
-    on another idle stream.  Either way it stops at `cap` queries (the first
-    request always fits) and at `budget_left` bytes of `nbytes(request)`,
+    `pending` holds work units (requests or request slices, each with
+    `queries` and `rid`) in arrival order, `free` the idle streams, `lane` the
+    stream kept for small units (at most `small_max` queries).  When the lane
+    is idle and a small unit waits, the launch takes pending small units only,
+    on the lane; otherwise it takes units in arrival order on another idle
+    stream.  A launch takes at most one unit of each request (the other slices
+    of a request wait for a later launch).  It stops at `cap` queries (the
+    first unit always fits) and at `budget_left` bytes of `nbytes(unit)`,
     which a launch may exceed only when nothing else runs (`busy` false).
-    Returns no requests when nothing can start."""
+    Returns no units when nothing can start."""
     small_lane = lane in free and any(r.queries.shape[0] <= small_max for r in pending)
     others = [x for x in free if x is not lane]
     if not small_lane and not others:
         return [], None, pending
     pending = deque(pending)
-    take, nq, nb, keep = [], 0, 0, deque()
+    take, nq, nb, keep, rids = [], 0, 0, deque(), set()
     while pending and (not take or nq + pending[0].queries.shape[0] <= cap):
         r = pending[0]
-        if small_lane and r.queries.shape[0] > small_max:
-            keep.append(pending.popleft())  # large: for the other streams
+        if (small_lane and r.queries.shape[0] > small_max) or r.rid in rids:
+            keep.append(pending.popleft())  # for the other streams, or a later launch
             continue
         b = nbytes(r)
         if nb + b > budget_left and (take or busy):
             break
         take.append(pending.popleft())
+        rids.add(r.rid)
         nq += r.queries.shape[0]
         nb += b
     keep.extend(pending)

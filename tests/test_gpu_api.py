@@ -187,3 +187,50 @@ def test_large_host_results_staged():
         t.join()
     for j in range(2):
         assert out[j][0].tobytes() == hd.tobytes() and np.array_equal(out[j][1], hi)
+
+
+def test_non_finite_queries_leave_outputs_untouched(small_tree):
+    # The finiteness check runs before the query (src/query.rs:46): a call that
+    # fails has written nothing into the caller's buffers, through the host
+    # entry point and the device one alike.
+    import ctypes
+
+    import torch
+
+    from kdtree._core import load_library
+
+    lib = load_library()
+    q = np.array([[0.1, 0.2], [np.nan, 0.0], [0.3, 0.3]])
+    out_d = np.full((3, 2), 7.0)
+    out_i = np.full((3, 2), 7, np.int64)
+    rc = lib.kdt_query(small_tree._handle, ctypes.c_void_p(q.ctypes.data), 3, 2, 2, 2.0, float("inf"), 0.0, 0,
+                       ctypes.c_void_p(out_d.ctypes.data), ctypes.c_void_p(out_i.ctypes.data))
+    assert rc == 7 and lib.kdt_last_error().decode() == "all coordinates must be finite"
+    assert np.all(out_d == 7.0) and np.all(out_i == 7)
+    qd = torch.from_numpy(q).cuda()
+    dd = torch.full((3, 2), 7.0, dtype=torch.float64, device="cuda")
+    di = torch.full((3, 2), 7, dtype=torch.int64, device="cuda")
+    rc = lib.kdt_query_device(small_tree._handle, ctypes.c_void_p(qd.data_ptr()), 3, 2, 2, 2.0, float("inf"), 0.0,
+                              0, ctypes.c_void_p(dd.data_ptr()), ctypes.c_void_p(di.data_ptr()),
+                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 7
+    torch.cuda.synchronize()
+    assert bool(torch.all(dd == 7.0)) and bool(torch.all(di == 7))
+
+
+def test_batch_item_overflow_checks(small_tree):
+    # The batch entry point checks each item like kdt_query_device: k beyond
+    # 32 bits and n_queries * k overflow are rejected before any launch.
+    import torch
+
+    from kdtree import forest as F
+
+    lib = F._lib()
+    q = torch.zeros((4, 2), dtype=torch.float64, device="cuda")
+    out = torch.zeros((4, 2), dtype=torch.float64, device="cuda")
+    idx = torch.zeros((4, 2), dtype=torch.int64, device="cuda")
+    for nq, k in ((4, 1 << 32), (1 << 62, 8)):
+        items = (F.BatchItem * 1)()
+        items[0] = F.BatchItem(small_tree._handle.value, q.data_ptr(), nq, 2, k, out.data_ptr(), idx.data_ptr(), 0, 0)
+        rc = lib.kdt_query_batch_device(items, 1, 2.0, float("inf"), 0.0, 0, None)
+        assert rc == 101 and "overflows" in lib.kdt_last_error().decode()

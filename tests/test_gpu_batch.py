@@ -29,6 +29,7 @@ def forest():
         host.append((data, queries, k, tree.leafsize))
     batch = F.Batch(items)
     yield batch, host
+    batch.close()
     for tree, _, _ in items:
         tree.free()
 
@@ -41,7 +42,7 @@ def check_all(batch, host):
         assert gd.cpu().numpy().tobytes() == od.tobytes(), f"item {j} distances"
 
 
-@pytest.mark.parametrize("mode", ["persistent", "per_tree"])
+@pytest.mark.parametrize("mode", ["persistent", "per_tree", "plan"])
 def test_batch_modes_match_oracle(forest, mode):
     import torch
 
@@ -61,6 +62,37 @@ def test_graph_replays_match_oracle(forest):
             oi.fill_(-1)
         torch.cuda.synchronize()
         batch.run("graph")
+        torch.cuda.synchronize()
+        check_all(batch, host)
+
+
+def test_no_order_call_captures_as_a_pure_enqueue(forest):
+    # kdt_query_batch_device with KDT_BATCH_NO_ORDER alone (no per-tree flag):
+    # every item on its own launch sequence on the caller's stream, nothing
+    # allocated or copied from the host, so a captured graph replays it.
+    import torch
+
+    from kdtree import forest as F
+
+    batch, host = forest
+    lib = F._lib()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        assert lib.kdt_query_batch_device(batch.arr, len(batch.items), 2.0, float("inf"), 0.0,
+                                          F.KDT_BATCH_NO_ORDER, F._stream_ptr(s)) == 0
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        assert lib.kdt_query_batch_device(batch.arr, len(batch.items), 2.0, float("inf"), 0.0,
+                                          F.KDT_BATCH_NO_ORDER, F._stream_ptr(None)) == 0
+    for replay in range(3):
+        for _, _, _, od, oi in batch.items:
+            od.fill_(-1.0)
+            oi.fill_(-1)
+        torch.cuda.synchronize()
+        g.replay()
         torch.cuda.synchronize()
         check_all(batch, host)
 
@@ -94,7 +126,7 @@ def test_item_screen_flags_match_oracle(forest):
     assert forced > 0, "no item the matrix-core screen applies to"
     try:
         b2 = F.Batch([(t, q, k) for t, q, k, _, _ in batch.items])
-        for mode in ("persistent", "per_tree", "graph", "graph", "graph"):
+        for mode in ("persistent", "per_tree", "plan", "graph", "graph", "graph"):
             for _, _, _, od, oi in b2.items:
                 od.fill_(-1.0)
                 oi.fill_(-1)
@@ -102,6 +134,7 @@ def test_item_screen_flags_match_oracle(forest):
             b2.run(mode)
             torch.cuda.synchronize()
             check_all(b2, host)
+        b2.close()
     finally:
         for t, f in zip(trees, saved):
             t.flags = f
@@ -168,7 +201,7 @@ def test_side_streams_join_into_caller_stream():
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
     try:
-        for mode in ("persistent", "per_tree"):
+        for mode in ("persistent", "per_tree", "plan"):
             print(f"mode {mode}", flush=True)
             with torch.cuda.stream(s):
                 for _, _, _, od, oi in batch.items:
@@ -188,5 +221,6 @@ def test_side_streams_join_into_caller_stream():
                 assert hd.numpy().tobytes() == od.tobytes(), f"{mode} item {j} distances"
     finally:
         torch.cuda.synchronize()
+        batch.close()
         for tree, _, _ in items:
             tree.free()

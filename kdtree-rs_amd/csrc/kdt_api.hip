@@ -205,27 +205,51 @@ hipError_t staged_copy(const std::vector<HostSeg>& segs, bool to_host, hipStream
     if (e_ != hipSuccess) return fail(KDT_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t nq, uint64_t ndim,
-               uint64_t k, double p, double max_distance, double eps, uint32_t flags, double* out_d,
-               int64_t* out_i, bool out_on_device, hipStream_t s) {
+}  // namespace
+
+// The argument checks of Tree::query in the reference's order (dimension,
+// src/lib.rs:74-84; k; shape; buffers and sizes at the ABI), then the query
+// parameters.  The scalar checks (eps, max_distance, p, src/query.rs:49-62)
+// come after the finiteness of the queries, which needs the device, so their
+// outcome is returned in scalar_rc / smsg for the caller to report in order.
+int kdt::query_args(const kdt_tree* tree, const double* q, uint64_t nq, uint64_t ndim, uint64_t k, double p,
+                    double max_distance, double eps, uint32_t flags, const void* out_d, const void* out_i,
+                    QueryParams& qp, int& scalar_rc, std::string& smsg) {
   if (!tree) return fail(KDT_ERR_ARGUMENT, "null tree");
   const kdt::Tree& t = tree->t;
-  if (ndim != t.d)  // src/lib.rs:74-84
+  if (ndim != t.d)
     return fail(KDT_ERR_DIMENSION_MISMATCH,
                 "dimension mismatch: expected " + std::to_string(t.d) + ", got " + std::to_string(ndim));
   if (k == 0) return fail(KDT_ERR_INVALID_K, "k must be greater than zero");
   if (nq == 0) return fail(KDT_ERR_INVALID_SHAPE, "queries must be a contiguous row-major matrix");
   if (!q || !out_d || !out_i) return fail(KDT_ERR_ARGUMENT, "null buffer");
   if (k > 0xFFFFFFFFull || nq > (~0ull) / k) return fail(KDT_ERR_ARGUMENT, "n_queries * k overflows");
-  kdt::QueryParams qp;
+  qp = kdt::QueryParams{};
   qp.k = k;
+  qp.nq = nq;
   qp.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   qp.screen_only = (flags & KDT_QUERY_SCREEN_ONLY) != 0;
   qp.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
   qp.group = kdt::requested_group(flags);
   qp.allow_order = (flags & KDT_QUERY_NO_ORDER) == 0;
+  scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
+  return KDT_OK;
+}
+
+int kdt::fail_api(int code, std::string msg) { return fail(code, std::move(msg)); }
+
+namespace {
+
+int query_impl(const kdt_tree* tree, const double* q, bool q_on_device, uint64_t nq, uint64_t ndim,
+               uint64_t k, double p, double max_distance, double eps, uint32_t flags, double* out_d,
+               int64_t* out_i, bool out_on_device, hipStream_t s) {
+  kdt::QueryParams qp;
+  int scalar_rc = 0;
   std::string smsg;
-  const int scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
+  const int arc = kdt::query_args(tree, q, nq, ndim, k, p, max_distance, eps, flags, out_d, out_i, qp, scalar_rc,
+                                  smsg);
+  if (arc) return arc;
+  const kdt::Tree& t = tree->t;
   if (q_on_device && out_on_device && (flags & KDT_QUERY_ASSUME_FINITE)) {
     if (scalar_rc) return fail(scalar_rc, smsg);
     qp.nq = nq;

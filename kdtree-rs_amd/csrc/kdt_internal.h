@@ -185,6 +185,18 @@ inline int requested_group(uint32_t flags) {
 // and messages of src/query.rs:49-62 and src/metric.rs:45-71; 0 or an error code.
 int query_params(double p, double max_distance, double eps, QueryParams& qp, std::string& msg);
 
+}  // namespace kdt
+struct kdt_tree;
+namespace kdt {
+// Argument checks of a query call (kdt_api.hip): 0 or an error code (whose
+// message is already set); scalar_rc / smsg carry the eps, max_distance and p
+// checks, reported after the finiteness check of the queries.
+int query_args(const kdt_tree* tree, const double* q, uint64_t nq, uint64_t ndim, uint64_t k, double p,
+               double max_distance, double eps, uint32_t flags, const void* out_d, const void* out_i,
+               QueryParams& qp, int& scalar_rc, std::string& smsg);
+// Sets the thread's last error and returns `code`.
+int fail_api(int code, std::string msg);
+
 // Rust's Display of an f64 (shortest round-trip digits, no exponent).
 std::string rust_f64(double x);
 

@@ -3,8 +3,9 @@
 // Replaces the plan dispatch of Tree::query (src/query.rs:68-75, with_plan!
 // src/kernel.rs:244-270): the metric and the row width pick a kernel once per
 // call.  Widths 1, 2, 3, 4, 8 and 16 (the reference's monomorphised list,
-// src/build.rs:60) have compiled kernels with the k list in registers when
-// k <= 64; everything else runs the runtime-width kernel whose k list is in LDS.
+// src/build.rs:60), 32 and 64 have compiled kernels with the k list in
+// registers when k <= 64; everything else runs the runtime-width kernel whose
+// k list is in LDS.
 
 #include <charconv>
 #include <cmath>
@@ -18,7 +19,13 @@ namespace {
 constexpr size_t LDS_LIST_LIMIT = 60000;  // bytes of dynamic LDS per block
 constexpr uint64_t ORDER_MIN_QUERIES = 8192; // smaller batches run in the given order
 
-bool compiled_width(uint64_t d) { return d == 1 || d == 2 || d == 3 || d == 4 || d == 8 || d == 16; }
+// Widths with compiled register-list kernels: the reference's monomorphised
+// list (src/build.rs:60) plus 32 and 64, the wide rows of the serving
+// workload, whose runtime-width kernel (one 64-lane group per query, half of
+// it idle on 32-row leaves, the k list in LDS) ran several times slower.
+bool compiled_width(uint64_t d) {
+  return d == 1 || d == 2 || d == 3 || d == 4 || d == 8 || d == 16 || d == 32 || d == 64;
+}
 
 }  // namespace
 
@@ -146,8 +153,15 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
   const bool reg_ok = qp.metric != M_LP && compiled_width(t.d) && qp.k <= 64;
   int rc = -1;
   if (reg_ok) {
-    const int G = group_width(qp.k, qp.group);
-    const bool screen = qp.fp32_screen && qp.metric == M_L2 && t.d <= 64;
+    int G = group_width(qp.k, qp.group);
+    // Wide rows: no 8-lane instances (compile time, LDS per block), and 32
+    // lanes by default for k <= 32 (a 32-row leaf in one pass: 64-d rows,
+    // 10^6 points, k = 10: 9.6 K queries/s vs 6.7 K with 16 lanes, 32-d 18.6 K
+    // vs 15.7 K).
+    if (t.d >= 32 && (G == 8 || (qp.group == 0 && G == 16))) G = qp.group == 0 ? 32 : 16;
+    // The traversal's FP32 leaf screen reads the FP32 leaf buckets, which the
+    // build keeps for rows of up to 16 coordinates.
+    const bool screen = qp.fp32_screen && qp.metric == M_L2 && t.pts32 != nullptr;
     switch (qp.metric) {
       case M_L1: rc = launch_knn_l1(a, G, s); break;
       case M_L2: rc = launch_knn_l2(a, G, screen, s); break;

@@ -58,6 +58,10 @@ namespace kdt {
 // STK_MAX otherwise; the runtime-width kernel keeps STK_MAX.
 constexpr int STK_MAX = 64;
 
+#ifndef KDT_WIDE_WAVES
+#define KDT_WIDE_WAVES 4
+#endif
+constexpr int WIDE_WAVES = KDT_WIDE_WAVES;
 // Waves per SIMD the register-list kernels are compiled for.  The descent's
 // dependent node loads need many waves to hide them; with enough waves the
 // traversal is bound by VALU issue (3-d rows at 8 waves: about 90% of the
@@ -67,7 +71,7 @@ constexpr int STK_MAX = 64;
 // of 16 take the registers they need (120: 4 waves; at 5 or 6 waves their leaf
 // loop spills).
 template <int DT>
-constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : DT == 8 ? 6 : 1; }
+constexpr int reg_min_waves() { return DT >= 1 && DT <= 4 ? 8 : DT == 8 ? 6 : DT >= 32 ? WIDE_WAVES : 1; }
 // Hits in one leaf chunk from which a group inserts them with one bitonic
 // merge instead of one by one: for groups of up to 16 lanes (all exchanges DPP
 // moves, 14 steps) the merge costs about five single inserts; wider groups
@@ -148,6 +152,7 @@ __device__ __forceinline__ double box_dist_d(const double* q, const double* __re
     double dist = 0.0;
     if (nch) {
       double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
       for (uint32_t ch = 0; ch < nch; ++ch) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -264,6 +269,10 @@ __device__ __forceinline__ double point_dist_d(const double* q, const double* __
     }
     double s = 0.0;
     uint32_t c = 0;
+    // One 8-coordinate chunk per iteration, not unrolled: unrolled over 32 or
+    // 64 coordinates, the compiler keeps every query coordinate live in
+    // registers across the leaf loop (128 VGPRs of them at 64 dimensions).
+#pragma unroll 1
     for (; c + 8 <= D; c += 8) {
       double w[8];
 #pragma unroll
@@ -518,9 +527,13 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
 
   uint64_t qi = 0;
   const double* qg = a.q;
-  double qr[DL];
-  float q32[DL];
-  const double* q = qr;
+  // The query's coordinates: in registers for rows of up to 16 coordinates,
+  // read from the group's LDS copy for wider rows (64 doubles would take 128
+  // VGPRs and force spills at any useful occupancy).
+  constexpr bool Q_IN_LDS = DT >= 32;
+  double qr[Q_IN_LDS ? 1 : DL];
+  float q32[Q_IN_LDS ? 1 : DL];
+  const double* q = Q_IN_LDS ? s_q : qr;
   double bound = a.cutoff, cell = 0.0, screen_e = 0.0;
   float t32 = INFINITY;
   bool screen_on = false;
@@ -531,7 +544,15 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
   auto start_query = [&]() {
     qg = a.q + qi * D;
     double qmax = 0.0;
-    if constexpr (DT > 0) {
+    if constexpr (Q_IN_LDS) {
+#pragma unroll 4
+      for (int c = 0; c < DT; ++c) {
+        const double v = qg[c];
+        s_q[c] = v;
+        axes[c] = 0.0;
+        qmax = fmax(qmax, fabs(v));
+      }
+    } else if constexpr (DT > 0) {
 #pragma unroll
       for (int c = 0; c < DT; ++c) {
         qr[c] = qg[c];
@@ -821,8 +842,18 @@ inline int launch_reg_width(const KArgs& a, hipStream_t s) {
     case 4: launch_reg<4, G, M, SCREEN>(a, s); return 0;
     case 8: launch_reg<8, G, M, SCREEN>(a, s); return 0;
     case 16: launch_reg<16, G, M, SCREEN>(a, s); return 0;
-    default: return -1;
+    default: break;
   }
+  // Wide rows (no FP32 leaf buckets, so no screened instances; no 8-lane
+  // groups): the Streamed recipe of point_dist_d, compiled for the width.
+  if constexpr (!SCREEN && G >= 16) {
+    switch (a.D) {
+      case 32: launch_reg<32, G, M, false>(a, s); return 0;
+      case 64: launch_reg<64, G, M, false>(a, s); return 0;
+      default: break;
+    }
+  }
+  return -1;
 }
 
 }  // namespace kdt

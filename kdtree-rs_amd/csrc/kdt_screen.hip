@@ -781,7 +781,11 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   se.pmax = t.pmax;
   se.cutoff = qp.cutoff;
   se.max_abs = t.max_abs;
-  k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(se);
+  {
+    KernelTimer tm(s);
+    k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(se);
+    tm.stop("k_screen_select", (double)nq);
+  }
 
   CheckArgs ca{};
   ca.q = trav.q;
@@ -800,7 +804,11 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   ca.n = (uint32_t)t.n;
   ca.leafsize = (uint32_t)std::min<uint64_t>(t.leafsize, 0xFFFFFFFFu);
   ca.cutoff = qp.cutoff;
-  k_screen_recheck<<<(uint32_t)nq, 64, (size_t)k * t.d * 8, s>>>(ca);
+  {
+    KernelTimer tm(s);
+    k_screen_recheck<<<(uint32_t)nq, 64, (size_t)k * t.d * 8, s>>>(ca);
+    tm.stop("k_screen_recheck", (double)nq);
+  }
 
   // Flagged queries: the exact traversal.
   KArgs fb = trav;

@@ -88,7 +88,7 @@ def test_c1_configuration():
     _query_case(data, queries, k=8)
 
 
-@pytest.mark.parametrize("dims", list(range(1, 18)) + [24, 33])
+@pytest.mark.parametrize("dims", list(range(1, 18)) + [24, 31, 32, 33, 63, 64])
 @pytest.mark.parametrize("p", [1.0, 2.0, np.inf])
 def test_row_widths(dims, p):
     # row-width-NN cases of tests/test_kdtree.py:153-163.
@@ -244,6 +244,36 @@ def test_lanes_per_query_do_not_change_results(group, dims, k):
     out_i = np.empty((queries.shape[0], k), dtype=np.int64)
     rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(queries.ctypes.data), queries.shape[0], dims, k, 2.0,
                              np.inf, 0.0, code << 4, ctypes.c_void_p(out_d.ctypes.data),
+                             ctypes.c_void_p(out_i.ctypes.data))
+    assert rc == 0, tree._lib.kdt_last_error().decode()
+    np.testing.assert_array_equal(out_i, oi)
+    assert out_d.tobytes() == od.tobytes()
+
+
+WIDE_CASES = [(k, g, 2.0, 0.0, np.inf) for k, g in [(1, 0), (10, 0), (10, 32), (16, 64), (32, 0), (64, 0), (8, 8)]] + [
+    (10, 0, 1.0, 0.0, np.inf), (10, 0, np.inf, 0.0, np.inf), (10, 0, 2.0, 0.5, np.inf), (10, 0, 2.0, 0.0, 9.0)]
+
+
+@pytest.mark.parametrize("dims", [32, 64])
+@pytest.mark.parametrize("k,group,p,eps,max_distance", WIDE_CASES)
+def test_wide_rows_compiled(dims, k, group, p, eps, max_distance):
+    # Rows of 32 and 64 coordinates run compiled register-list kernels (the
+    # Streamed recipe, src/kernel.rs:177-188, :296-303); 8-lane requests take
+    # 16 lanes.  9000 queries: the leaf-ordered path too.  Float32-valued
+    # Gaussian data as in the serving workload.
+    import ctypes
+
+    rng = np.random.default_rng(300 + dims + k)
+    data = rng.normal(size=(6000, dims)).astype(np.float32).astype(np.float64)
+    queries = rng.normal(size=(9000, dims)).astype(np.float32).astype(np.float64)
+    md = None if np.isinf(max_distance) else max_distance
+    od, oi = OracleTree(data, 32).query(queries, k=k, p=p, eps=eps, max_distance=md, threads=8)
+    tree = KDTree(data, leafsize=32)
+    code = {0: 0, 8: 1, 16: 2, 32: 3, 64: 4}[group]
+    out_d = np.empty((queries.shape[0], k))
+    out_i = np.empty((queries.shape[0], k), dtype=np.int64)
+    rc = tree._lib.kdt_query(tree._h, ctypes.c_void_p(queries.ctypes.data), queries.shape[0], dims, k, p,
+                             max_distance, eps, code << 4, ctypes.c_void_p(out_d.ctypes.data),
                              ctypes.c_void_p(out_i.ctypes.data))
     assert rc == 0, tree._lib.kdt_last_error().decode()
     np.testing.assert_array_equal(out_i, oi)

@@ -153,6 +153,24 @@ int kdt_query_device(const kdt_tree* tree, const double* d_queries, uint64_t n_q
                      uint64_t ndim, uint64_t k, double p, double max_distance, double eps,
                      uint32_t flags, double* d_out_dist, int64_t* d_out_idx, void* stream);
 
+/* kdt_query_device without any allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, checks and
+ * results as kdt_query_device; the call's scratch (finiteness flag, leaf-order
+ * sort, matrix-core screen lists) comes from one fixed device array of
+ * KDT_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load on that GPU
+ * until the process ends, and the call makes no hipMalloc, hipFree or other
+ * allocating call.  A query whose scratch would exceed that size is rejected
+ * with KDT_ERR_ARGUMENT before any launch (e.g. more than about 80,000 queries
+ * on the 64-d matrix-core screen, or about 5 million through the leaf-order
+ * sort).  Every call on one GPU shares the array and overwrites it: calls
+ * whose work overlaps on one GPU (several streams or threads) race, so the
+ * caller orders them (e.g. one stream). */
+#define KDT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int kdt_query_device_noalloc(const kdt_tree* tree, const double* d_queries, uint64_t n_queries,
+                             uint64_t ndim, uint64_t k, double p, double max_distance, double eps,
+                             uint32_t flags, double* d_out_dist, int64_t* d_out_idx, void* stream);
+
 /* Many trees and query sets in one persistent launch.  No reference
  * counterpart (the reference answers one matrix against one tree per call,
  * src/query.rs:29-78); each item's result is identical to kdt_query_device on

@@ -18,11 +18,14 @@
 //      A = rows [0,lt) with key >= pivot in ascending order, B = rows [lt,len)
 //      with key < pivot ascending; A[i] swaps with B[m-1-i].  The pairs are
 //      disjoint, so all swaps run in parallel.  The fix-up loop that pulls
-//      pivot-equal rows into [lt,mid) is a stable partition of the window
-//      [lt, e] (e = position of the (mid-lt)-th pivot-equal row): equal rows
-//      keep their order and move to [lt,mid), the others keep their order and
-//      follow.  That is what the sequential swap loop produces, because the
-//      rows it has passed over form a queue that rotates one slot per swap.
+//      pivot-equal rows into [lt,mid) (src/build.rs:303-313) is not a stable
+//      partition: its t-th swap exchanges position lt+t-1 with s_t, the t-th
+//      pivot-equal position >= lt, so pivot-equal rows land in [lt,mid) in
+//      order, rows already in [mid, s_need] stay, and a row at P in [lt,mid)
+//      is carried along P -> g(P) -> g(g(P)) ... (g(lt+t-1) = s_t) to the
+//      first position >= mid: the rows the loop passes over rotate.  g is
+//      increasing, so the chains are resolved in parallel by pointer jumping
+//      (k_fix_* below).
 // Rows move physically (with their original index), as in the reference, so
 // every node's rows stay one contiguous row-major block.
 //

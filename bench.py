@@ -133,7 +133,8 @@ def traversal_bytes(d: int, k: int, E: float, V: float, X: float) -> float:
 
 def kernel_summary(times: dict) -> dict:
     return {name: {"launches": v["launches"], "ms_total": v["ms"], "ms_avg": v["ms"] / max(1, v["launches"]),
-                   "work": v["work"]} for name, v in sorted(times.items(), key=lambda kv: -kv[1]["ms"])}
+                   "busy_ms": v["busy_ms"], "work": v["work"]}
+            for name, v in sorted(times.items(), key=lambda kv: -kv[1]["ms"])}
 
 
 # --------------------------------------------------------------- plain GPU kernel
@@ -180,7 +181,7 @@ def c2_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     wall = time.perf_counter() - t0
     F.timing_enable(False)
     times = F.timing_read()
-    trav = max(times.items(), key=lambda kv: kv[1]["ms"])
+    trav = max(((n, v) for n, v in times.items() if n != "*"), key=lambda kv: kv[1]["ms"])
     kernel_ms = trav[1]["ms"] / trav[1]["launches"]
 
     otree = OracleTree(data_h, ls)
@@ -258,7 +259,7 @@ def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     bq = traversal_bytes(d, k, E, V, X)
 
     def roof(times):
-        name, v = max(times.items(), key=lambda kv: kv[1]["ms"])
+        name, v = max(((n, v) for n, v in times.items() if n != "*"), key=lambda kv: kv[1]["ms"])
         ms = v["ms"] / v["launches"]
         ach = bq * nq / (ms / 1e3) / 1e9
         return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
@@ -374,8 +375,9 @@ def run_serve(args):
     times = F.timing_read()
     lat = np.concatenate([[r.done - r.arrival for r in rs] for rs in runs]) * 1e3
     served_qps = total_q * args.steps / wall
-    scr = times.get("k_screen<32>", {"launches": 0, "ms": 0.0, "work": 0.0})
+    scr = times.get("k_screen<32>", {"launches": 0, "ms": 0.0, "work": 0.0, "busy_ms": -1.0})
     achieved = scr["work"] / (scr["ms"] / 1e3) / 1e12 if scr["ms"] else 0.0
+    achieved_busy = scr["work"] / (scr["busy_ms"] / 1e3) / 1e12 if scr["busy_ms"] > 0 else None
 
     # Latency at lower loads.
     loads = {}
@@ -441,11 +443,12 @@ def run_serve(args):
 
     cpu = None
     if not args.no_cpu:
-        # The oracle on the plain sample's requests, each cut to 32 queries,
-        # over the trees they use (built beforehand in parallel, not timed).
+        # The oracle on the first 12 requests of the plain sample, each cut to
+        # 16 queries, over the trees they use (built beforehand in parallel,
+        # not timed): about 10-30 s of CPU work.
         log("cpu baseline")
         threads = host_threads()
-        cut = [(r.tree, np.ascontiguousarray(r.queries[:32])) for r in sample]
+        cut = [(r.tree, np.ascontiguousarray(r.queries[:16])) for r in sample[:12]]
         need = sorted({t for t, _ in cut})
 
         def build_oracle(t):
@@ -457,7 +460,7 @@ def run_serve(args):
         units = sum(q.shape[0] for _, q in cut)
         cpu = cpu_timed(lambda: [otrees[t].query(q, k=k, threads=threads) for t, q in cut], units)
         cpu.update(unit="queries/s", cores=threads, kind="port", cpu_model=cpu_model(), host_cpus=os.cpu_count(),
-                   sample=f"{len(cut)} requests of the plain sample on {len(need)} trees, at most 32 queries each "
+                   sample=f"{len(cut)} requests of the plain sample on {len(need)} trees, at most 16 queries each "
                           f"({units} queries), k={k}, {threads} threads (oracle = C++ restatement of the reference "
                           "query, g++ -O3 -ffp-contract=off)")
         del otrees
@@ -487,13 +490,18 @@ def run_serve(args):
                      "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_screen<32>",
                      "launches": scr["launches"], "kernel_ms_avg": scr["ms"] / max(1, scr["launches"]),
                      "flops_per_launch_avg": scr["work"] / max(1, scr["launches"]),
+                     "achieved_over_busy_time": achieved_busy,
+                     "busy_ms": scr["busy_ms"],
                      "what": "2 d FLOPs per (query, point) pair of the 64-d matrix-core screen over the timed "
                              "serving steps / the summed HIP-event durations of its launches (each on its own "
-                             "stream; concurrent launches share the device)"},
+                             "stream; launches on several streams overlap and share the device, so each one's "
+                             "duration includes the sharing); achieved_over_busy_time divides by the union of "
+                             "the launches' intervals instead"},
         "roofline_screen_alone": {"achieved": alone_tf, "frac": alone_tf / BF16_PEAK_TFLOPS,
                                   "kernel_ms_avg": sa["ms"] / max(1, sa["launches"]), "call_ms": alone_s * 1e3,
                                   "what": "k_screen<32> on the largest 64-d tree, 65,536 queries, one stream"},
         "kernel_times": kernel_summary(times),
+        "device_busy_frac": (times["*"]["busy_ms"] / 1e3 / wall) if "*" in times and times["*"]["busy_ms"] > 0 else None,
         "served_saturation": sat, "served_at_load": loads,
         "d16_choice_ms_per_4096_queries": tune,
         "d16_matrix_screen": bool(serve_flags & F.KDT_QUERY_MATRIX_SCREEN),

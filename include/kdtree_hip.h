@@ -206,17 +206,22 @@ void kdt_batch_plan_free(kdt_batch_plan* plan);            /* waits for the devi
 /* Kernel timing, a measurement aid with no reference counterpart.  While it is
  * enabled (kdt_timing_enable(1); returns the previous state), every launch of
  * the main kernels -- traversal (knn_reg, knn_lds, knn_batch) and matrix-core
- * screen (k_screen) -- is bracketed by HIP events on the stream it runs on.
- * kdt_timing_read waits for those events and returns per kernel name the
- * launches, their summed duration and their summed algorithmic work (FLOPs of
- * the screen's contraction, 2 ndim per (query, point) pair; queries for a
- * traversal) since the last read; *count = kernel names (entries beyond
- * `capacity` are dropped).  Not for use under graph capture. */
+ * screen (k_screen, k_screen_select, k_screen_recheck) -- is bracketed by HIP
+ * events on the stream it runs on.  kdt_timing_read waits for those events and
+ * returns per kernel name the launches, their summed duration, their summed
+ * algorithmic work (FLOPs of the screen's contraction, 2 ndim per (query,
+ * point) pair; queries for the others) and the kernel's busy time, the length
+ * of the union of its launches' intervals (launches overlap when several
+ * streams run them; -1 when unknown), since the last read, then one entry
+ * named "*" for all of them together (its busy time: the device ran at least
+ * one timed kernel); *count = entries (those beyond `capacity` are dropped).
+ * Not for use under graph capture. */
 typedef struct kdt_kernel_time {
   char name[48];
   uint64_t launches;
   double ms;
   double work;
+  double busy_ms;
 } kdt_kernel_time;
 int kdt_timing_enable(int on);
 int kdt_timing_read(kdt_kernel_time* out, uint32_t capacity, uint32_t* count);

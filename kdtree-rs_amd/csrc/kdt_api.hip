@@ -35,6 +35,7 @@ struct TimedLaunch {
 std::atomic<bool> g_timing{false};
 std::mutex g_timing_mu;
 std::vector<TimedLaunch> g_timed;
+hipEvent_t g_timing_ref = nullptr;  // recorded when timing is enabled: the origin of the busy intervals
 }  // namespace
 
 void kdt::set_last_error(std::string msg) { g_last_error = std::move(msg); }
@@ -454,21 +455,34 @@ void kdt_free(kdt_tree* tree) {
 
 const char* kdt_last_error(void) { return g_last_error.c_str(); }
 
-int kdt_timing_enable(int on) { return g_timing.exchange(on != 0) ? 1 : 0; }
+int kdt_timing_enable(int on) {
+  if (on) {
+    std::lock_guard<std::mutex> g(g_timing_mu);
+    if (!g_timing_ref) (void)hipEventCreate(&g_timing_ref);
+    if (g_timing_ref) (void)hipEventRecord(g_timing_ref, nullptr);
+  }
+  return g_timing.exchange(on != 0) ? 1 : 0;
+}
 
 int kdt_timing_read(kdt_kernel_time* out, uint32_t capacity, uint32_t* count) {
   if (!count || (!out && capacity)) return fail(KDT_ERR_ARGUMENT, "null argument");
   std::vector<TimedLaunch> recs;
+  hipEvent_t ref = nullptr;
   {
     std::lock_guard<std::mutex> g(g_timing_mu);
     recs.swap(g_timed);
+    ref = g_timing_ref;
   }
   std::vector<kdt_kernel_time> agg;
+  std::vector<std::vector<std::pair<float, float>>> spans;  // per name: [start, end) from `ref`
   hipError_t err = hipSuccess;
   for (TimedLaunch& r : recs) {
-    float ms = 0.0f;
+    float ms = 0.0f, t0 = 0.0f, t1 = 0.0f;
     hipError_t e = hipEventSynchronize(r.b);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.a, r.b);
+    bool have_span = false;
+    if (e == hipSuccess && ref)
+      have_span = hipEventElapsedTime(&t0, ref, r.a) == hipSuccess && hipEventElapsedTime(&t1, ref, r.b) == hipSuccess;
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
     if (e != hipSuccess) {
@@ -481,10 +495,66 @@ int kdt_timing_read(kdt_kernel_time* out, uint32_t capacity, uint32_t* count) {
       kdt_kernel_time z{};
       std::strncpy(z.name, r.name.c_str(), sizeof z.name - 1);
       agg.push_back(z);
+      spans.emplace_back();
     }
     agg[j].launches += 1;
     agg[j].ms += ms;
     agg[j].work += r.work;
+    if (have_span) spans[j].push_back({t0, t1});
+  }
+  // Busy time per kernel: the length of the union of its launches' intervals
+  // (launches on several streams overlap; each one's own duration then
+  // includes the time it shared the device).
+  for (size_t j = 0; j < agg.size(); ++j) {
+    std::vector<std::pair<float, float>>& v = spans[j];
+    std::sort(v.begin(), v.end());
+    double busy = 0.0, cs = 0.0, ce = -1.0;
+    bool open = false;
+    for (const auto& iv : v) {
+      if (!open || iv.first > ce) {
+        if (open) busy += ce - cs;
+        cs = iv.first;
+        ce = iv.second;
+        open = true;
+      } else if (iv.second > ce) {
+        ce = iv.second;
+      }
+    }
+    if (open) busy += ce - cs;
+    agg[j].busy_ms = v.size() == agg[j].launches ? busy : -1.0;
+  }
+  // "*": every timed kernel together (the union of all intervals): the time
+  // the device ran at least one of them.
+  if (!agg.empty()) {
+    std::vector<std::pair<float, float>> all;
+    bool complete = true;
+    for (size_t j = 0; j < agg.size(); ++j) {
+      complete = complete && spans[j].size() == agg[j].launches;
+      all.insert(all.end(), spans[j].begin(), spans[j].end());
+    }
+    std::sort(all.begin(), all.end());
+    double busy = 0.0, cs = 0.0, ce = -1.0;
+    bool open = false;
+    kdt_kernel_time z{};
+    std::strncpy(z.name, "*", sizeof z.name - 1);
+    for (const auto& iv : all) {
+      if (!open || iv.first > ce) {
+        if (open) busy += ce - cs;
+        cs = iv.first;
+        ce = iv.second;
+        open = true;
+      } else if (iv.second > ce) {
+        ce = iv.second;
+      }
+    }
+    if (open) busy += ce - cs;
+    for (const kdt_kernel_time& a : agg) {
+      z.launches += a.launches;
+      z.ms += a.ms;
+      z.work += a.work;
+    }
+    z.busy_ms = complete ? busy : -1.0;
+    agg.push_back(z);
   }
   for (size_t j = 0; j < agg.size() && j < capacity; ++j) out[j] = agg[j];
   *count = (uint32_t)agg.size();

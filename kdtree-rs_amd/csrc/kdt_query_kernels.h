@@ -90,6 +90,7 @@ struct KArgs {
   const double* __restrict__ q;
   const uint32_t* __restrict__ order;  // processing order of the queries (null: as given)
   const uint8_t* __restrict__ only;    // non-null: run only the queries qi with only[qi] != 0
+  const uint32_t* __restrict__ only_count;  // with `only`: how many are set (0: the grid exits at once)
   uint32_t stk;                        // stack entries per group (2 x tree depth, at least 2)
   double* __restrict__ out_d;
   int64_t* __restrict__ out_i;
@@ -766,6 +767,7 @@ __device__ __forceinline__ void knn_body(const KArgs& a, double* axes, double* s
   const uint32_t full = gridDim.x & ~7u;
   const uint32_t blk = blockIdx.x < full ? (blockIdx.x & 7u) * (full >> 3) + (blockIdx.x >> 3) : blockIdx.x;
   uint64_t qs = (uint64_t)blk * gpb + grp;
+  if (a.only && a.only_count && *a.only_count == 0) return;  // nothing left to the traversal (uniform)
   bool first = true;
   auto next = [&](uint64_t& qi) -> bool {
     for (;;) {

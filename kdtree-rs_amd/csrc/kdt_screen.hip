@@ -100,42 +100,77 @@ struct SArgs {
 template <int KS>
 constexpr int screen_waves() { return KS <= 16 ? 4 : 8; }
 
-// Point tile t, 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS], KS / 2 uint4),
-// staged global -> registers -> LDS by the whole workgroup of SW = screen_waves
-// waves: thread u copies uint4 u, u + 64 SW, ... of the tile (coalesced), and
-// threads 0..31 its split |p~|^2.
-// (Nothing is read past the tile: TU and 32 bound every index.)
+// Point tiles per pipeline step: the TS tiles of a step are staged together
+// and contracted between two barriers.  Narrow rows make short tiles (4
+// matrix instructions for rows of up to 16 columns against 13 for 64), so a
+// barrier and a staging round per tile leave the matrix cores idle; a step of
+// several tiles amortises them.
+#ifndef KDT_SCREEN_TS8
+#define KDT_SCREEN_TS8 1
+#endif
+#ifndef KDT_SCREEN_TS16
+#define KDT_SCREEN_TS16 1
+#endif
+#ifndef KDT_SCREEN_TS32
+#define KDT_SCREEN_TS32 1
+#endif
+template <int KS>
+constexpr int screen_tiles_per_step() { return KS <= 8 ? KDT_SCREEN_TS8 : KS <= 16 ? KDT_SCREEN_TS16 : KDT_SCREEN_TS32; }
+
+// One step: TS point tiles, each 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS],
+// KS / 2 uint4), staged global -> registers -> LDS by the whole workgroup of
+// SW = screen_waves waves: thread u copies uint4 u, u + 64 SW of the step
+// (coalesced within each tile), and threads 0 .. 32 TS - 1 the split |p~|^2.
+// (Nothing is read past a tile: TU, SU and the clamps bound every index.)
 template <int KS>
 struct Stage {
   static constexpr int RU = KS / 2;                          // uint4 per row
+  static constexpr int RS = RU + 1;                          // LDS row stride (one uint4 of padding)
   static constexpr int TU = 32 * RU;                         // uint4 per tile
+  static constexpr int TS = screen_tiles_per_step<KS>();
+  static constexpr int SU = TS * TU;                         // uint4 per step
   static constexpr int SW = screen_waves<KS>();
-  static constexpr int PT = (TU + 64 * SW - 1) / (64 * SW);  // uint4 per thread
+  static constexpr int NT = 64 * SW;                         // threads per workgroup
+  static constexpr int PT = (SU + NT - 1) / NT;              // uint4 per thread
   static_assert(PT <= 2, "two staging registers per thread");
+  static_assert(TS * 32 <= NT, "one thread per split norm of the step");
 };
 
 // The stage is three plain registers (v0, v1, pn), not an aggregate: an
 // aggregate carried through the loop ends up on the stack.  Loads are
-// unconditional, at clamped (in-bounds) indices; stage_store keeps u < TU.
+// unconditional, at clamped (in-bounds) indices: the step's local tiles
+// lt0 .. lt0 + TS - 1 are clamped to the slice's last tile t1 - 1 (a harmless
+// re-read), and local tile i is global tile tb + i ts.
 template <int KS>
-__device__ __forceinline__ void stage_load(const SArgs& a, uint32_t t, uint32_t tid, uint4& v0, uint4& v1,
-                                           uint32_t& pn) {
-  const uint4* src = reinterpret_cast<const uint4*>(a.rows_bf) + (size_t)t * Stage<KS>::TU;
-  v0 = src[min(tid, (uint32_t)Stage<KS>::TU - 1)];
-  if (Stage<KS>::PT > 1) v1 = src[min(tid + 64 * Stage<KS>::SW, (uint32_t)Stage<KS>::TU - 1)];
-  pn = a.pnorm_bf[(size_t)t * 32 + (tid & 31)];
+__device__ __forceinline__ void stage_load(const SArgs& a, uint32_t lt0, uint32_t t1, uint32_t tb, uint32_t ts,
+                                           uint32_t tid, uint4& v0, uint4& v1, uint32_t& pn) {
+  using St = Stage<KS>;
+  const uint4* src = reinterpret_cast<const uint4*>(a.rows_bf);
+  auto gtile = [&](uint32_t i) -> size_t { return (size_t)tb + (size_t)min(lt0 + i, t1 - 1) * ts; };
+  {
+    const uint32_t u = min(tid, (uint32_t)St::SU - 1);
+    v0 = src[gtile(u / St::TU) * St::TU + u % St::TU];
+  }
+  if (St::PT > 1) {
+    const uint32_t u = min(tid + (uint32_t)St::NT, (uint32_t)St::SU - 1);
+    v1 = src[gtile(u / St::TU) * St::TU + u % St::TU];
+  }
+  const uint32_t pi = min(tid, (uint32_t)St::TS * 32 - 1);
+  pn = a.pnorm_bf[gtile(pi / 32) * 32 + pi % 32];
 }
 
 // LDS rows are padded by one uint4 (stride KS / 2 + 1 uint4), so the 32 lanes
-// of a half-wave reading 32 different rows hit distinct banks.
+// of a half-wave reading 32 different rows hit distinct banks; tile i of the
+// step starts at rows + i 32 RS and pn + i 32.
 template <int KS>
 __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, uint32_t pnv, uint32_t tid,
                                             uint4* rows, uint32_t* pn) {
-  constexpr int RU = Stage<KS>::RU;
-  if (tid < (uint32_t)Stage<KS>::TU) rows[(tid / RU) * (RU + 1) + tid % RU] = v0;
-  const uint32_t u = tid + 64 * Stage<KS>::SW;
-  if (Stage<KS>::PT > 1 && u < (uint32_t)Stage<KS>::TU) rows[(u / RU) * (RU + 1) + u % RU] = v1;
-  if (tid < 32) pn[tid] = pnv;
+  using St = Stage<KS>;
+  auto at = [](uint32_t u) { return (u / St::TU) * 32 * St::RS + ((u % St::TU) / St::RU) * St::RS + u % St::RU; };
+  if (tid < (uint32_t)St::SU) rows[at(tid)] = v0;
+  const uint32_t u = tid + St::NT;
+  if (St::PT > 1 && u < (uint32_t)St::SU) rows[at(u)] = v1;
+  if (tid < (uint32_t)St::TS * 32) pn[tid] = pnv;
 }
 
 // One wave: 32 queries (B operand, columns) against 32-row tiles of points
@@ -161,9 +196,13 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, ui
 // each list holds the 16 smallest A of its lane's points, as with immediate
 // inserts.  IB = 16 slots for rows of <= 16 columns and 12 above (16 would
 // leave room for only one workgroup per CU beside the larger point tiles).
+#ifndef KDT_SCREEN_IB8
+#define KDT_SCREEN_IB8 16
+#endif
 template <int KS>
-constexpr uint32_t insert_slots() { return KS <= 8 ? 16 : 12; }
+constexpr uint32_t insert_slots() { return KS <= 8 ? KDT_SCREEN_IB8 : 12; }
 constexpr int CHK = 8;
+static_assert(KDT_SCREEN_IB8 > CHK, "a buffer holds more than one check interval");
 
 template <int KS>
 struct Screen {
@@ -263,9 +302,10 @@ struct Screen {
 template <int KS>
 __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int SW = screen_waves<KS>();
-  constexpr int RS = KS / 2 + 1;
-  __shared__ uint4 s_rows[2][32 * RS];
-  __shared__ __align__(16) uint32_t s_pn[2][32];
+  constexpr int RS = Stage<KS>::RS;
+  constexpr int TS = Stage<KS>::TS;
+  __shared__ uint4 s_rows[2][TS * 32 * RS];
+  __shared__ __align__(16) uint32_t s_pn[2][TS * 32];
   __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
@@ -324,37 +364,43 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     ts = 1;
     t1 = tb < a.n_tiles ? min(a.tiles_per_slice, a.n_tiles - tb) : 0;
   }
-  const uint32_t t0 = 0;
 #define KDT_TILE(i) (tb + (i) * ts)
-  // Tile t sits in LDS stage (t - t0) & 1 while it is contracted; tile t + 1
-  // is written to the other stage in the same iteration (its readers, tile
-  // t - 1, finished before the barrier that ended the previous iteration) and
-  // tile t + 2 is in flight to registers.  x and y alternate as the register
-  // stage of tiles t + 1 and t + 2.  Every path issues the same loads (tile
-  // indices clamped to t1 - 1, a harmless re-read at the end of the slice), so
-  // the compiler's wait before each LDS store covers only the tile it stores
-  // and the next tile's loads stay in flight.
+  // Steps of TS local tiles: step q holds local tiles q TS .. q TS + TS - 1
+  // (those from t1 on are re-reads of tile t1 - 1 and are not contracted).
+  // Step q sits in LDS stage q & 1 while it is contracted; step q + 1 is
+  // written to the other stage in the same iteration (its readers, step
+  // q - 1, finished before the barrier that ended the previous iteration) and
+  // step q + 2 is in flight to registers.  x and y alternate as the register
+  // stage of steps q + 1 and q + 2.  Every path issues the same loads (indices
+  // clamped), so the compiler's wait before each LDS store covers only the
+  // step it stores and the next step's loads stay in flight.
   // (A macro rather than a lambda: the two register stages must stay named
   // registers, never a stack object behind a reference.)
-#define KDT_SCREEN_STEP(T, B, N0, N1, NP)                                                 \
-  do {                                                                                    \
-    if ((T) + 1 < t1) stage_store<KS>(N0, N1, NP, tid, s_rows[(B) ^ 1], s_pn[(B) ^ 1]); \
-    stage_load<KS>(a, KDT_TILE(min((T) + 3, t1 - 1)), tid, N0, N1, NP);                 \
-    if (active && (T) < t1) sc.tile(KDT_TILE(T), j, h, s_rows[B], s_pn[B]);               \
-    __syncthreads();                                                                      \
+  const uint32_t nsteps = (t1 + TS - 1) / TS;
+#define KDT_SCREEN_STEP(Q, B, N0, N1, NP)                                                        \
+  do {                                                                                           \
+    if ((Q) + 1 < nsteps) stage_store<KS>(N0, N1, NP, tid, s_rows[(B) ^ 1], s_pn[(B) ^ 1]);      \
+    stage_load<KS>(a, min((Q) + 3, nsteps - 1) * TS, t1, tb, ts, tid, N0, N1, NP);              \
+    if (active && (Q) < nsteps) {                                                                \
+      _Pragma("unroll") for (int i_ = 0; i_ < TS; ++i_) {                                        \
+        const uint32_t lt_ = (Q) * TS + i_;                                                      \
+        if (lt_ < t1) sc.tile(KDT_TILE(lt_), j, h, s_rows[B] + i_ * 32 * RS, s_pn[B] + i_ * 32); \
+      }                                                                                          \
+    }                                                                                            \
+    __syncthreads();                                                                             \
   } while (0)
-  if (t0 < t1) {  // uniform over the workgroup
+  if (nsteps > 0) {  // uniform over the workgroup
     uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
     uint32_t xp = 0, yp = 0;
-    stage_load<KS>(a, KDT_TILE(t0), tid, x0, x1, xp);
+    stage_load<KS>(a, 0, t1, tb, ts, tid, x0, x1, xp);
     stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
-    stage_load<KS>(a, KDT_TILE(min(t0 + 1, t1 - 1)), tid, x0, x1, xp);
+    stage_load<KS>(a, min(1u, nsteps - 1) * TS, t1, tb, ts, tid, x0, x1, xp);
     __builtin_amdgcn_sched_barrier(0);  // keep the issue order of the loop: x's loads, then y's
-    stage_load<KS>(a, KDT_TILE(min(t0 + 2, t1 - 1)), tid, y0, y1, yp);
+    stage_load<KS>(a, min(2u, nsteps - 1) * TS, t1, tb, ts, tid, y0, y1, yp);
     __syncthreads();
-    for (uint32_t t = t0; t < t1; t += 2) {
-      KDT_SCREEN_STEP(t, 0, x0, x1, xp);
-      KDT_SCREEN_STEP(t + 1, 1, y0, y1, yp);
+    for (uint32_t q = 0; q < nsteps; q += 2) {
+      KDT_SCREEN_STEP(q, 0, x0, x1, xp);
+      KDT_SCREEN_STEP(q + 1, 1, y0, y1, yp);
     }
   }
 #undef KDT_SCREEN_STEP
@@ -377,6 +423,7 @@ struct SelArgs {
   uint32_t* __restrict__ cnt;
   uint32_t* __restrict__ cand;
   uint8_t* __restrict__ flag;
+  uint32_t* __restrict__ nflag;  // queries flagged so far (cleared before the select)
   uint64_t nq;
   uint32_t D, k, lists;
   double pmax, cutoff, max_abs;
@@ -496,6 +543,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   if (lane == 0) {
     a.cnt[qi] = good ? total : 0;
     a.flag[qi] = good ? 0 : 1;
+    if (!good) atomicAdd(a.nflag, 1u);
   }
 }
 
@@ -508,6 +556,7 @@ struct CheckArgs {
   const uint32_t* __restrict__ cnt;
   const uint32_t* __restrict__ cand;
   uint8_t* __restrict__ flag;
+  uint32_t* __restrict__ nflag;
   double* __restrict__ out_d;
   int64_t* __restrict__ out_i;
   uint64_t nq;
@@ -539,6 +588,13 @@ __device__ __forceinline__ bool in_full_chunk(uint32_t p, uint32_t n, uint32_t l
   return p - start < len / P * P;
 }
 
+// Marks query qi for the traversal (called once per query, by one lane: the
+// recheck returns at once for a query the select already flagged).
+__device__ __forceinline__ void flag_query(const CheckArgs& a, uint64_t qi) {
+  a.flag[qi] = 1;
+  atomicAdd(a.nflag, 1u);
+}
+
 // One wave per query: FP64 recheck of the candidates, top-k, certificate.
 // Dynamic LDS: k x D doubles, the cell-bound axes of each certificate walk.
 __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
@@ -552,7 +608,7 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   if (a.flag[qi]) return;
   const uint32_t c = a.cnt[qi];
   if (c > CAND_CAP) {
-    if (lane == 0) a.flag[qi] = 1;
+    if (lane == 0) flag_query(a, qi);
     return;
   }
   const double* q = a.q + qi * D;
@@ -564,7 +620,7 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
     bad_pos = pos >= a.n;
   }
   if (__ballot(bad_pos)) {  // never for a list built by k_screen_select
-    if (lane == 0) a.flag[qi] = 1;
+    if (lane == 0) flag_query(a, qi);
     return;
   }
   if (lane < c) {
@@ -654,7 +710,7 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
     }
   }
   if (__ballot(!ok)) {
-    if (lane == 0) a.flag[qi] = 1;
+    if (lane == 0) flag_query(a, qi);
     return;
   }
   // Box gates, all paths at once: offsets of each lane's list, then lane w
@@ -677,7 +733,7 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
     bad |= !(box_dist_d<M_L2, 0>(q, a.boxes + (size_t)child * 2 * D, D, 2.0) <= dcert);
   }
   if (__ballot(bad)) {
-    if (lane == 0) a.flag[qi] = 1;
+    if (lane == 0) flag_query(a, qi);
     return;
   }
   if (lane < a.k) {  // write_results, src/query.rs:382-400
@@ -714,7 +770,7 @@ size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp) {
   const uint64_t slices = screen_slices(t, qp, &per);
   const uint64_t nq = qp.nq;
   return 2 * scratch_piece(nq * 2 * slices * KP * 4) + scratch_piece(nq * 4) + scratch_piece(nq * CAND_CAP * 4) +
-         scratch_piece(nq);
+         scratch_piece(nq) + scratch_piece(4);
 }
 
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
@@ -727,16 +783,18 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   const uint64_t slices = screen_slices(t, qp, &per);
 
   // Scratch: every piece is written in full before it is read (the lists by
-  // k_screen, counts, candidates and flags by k_screen_select), so nothing
-  // needs clearing.
+  // k_screen, counts, candidates and flags by k_screen_select), except the
+  // flagged-query count, which a kernel clears before the select.
   float* part_a = nullptr;
   uint32_t *part_i = nullptr, *cnt = nullptr, *cand = nullptr;
   uint8_t* flag = nullptr;
+  uint32_t* nflag = nullptr;
   hipError_t e = sc.get(&part_a, nq * 2 * slices * KP);
   if (e == hipSuccess) e = sc.get(&part_i, nq * 2 * slices * KP);
   if (e == hipSuccess) e = sc.get(&cnt, nq);
   if (e == hipSuccess) e = sc.get(&cand, nq * CAND_CAP);
   if (e == hipSuccess) e = sc.get(&flag, nq);
+  if (e == hipSuccess) e = sc.get(&nflag, 1);
   if (e != hipSuccess) {
     msg = std::string("screen scratch: ") + hipGetErrorString(e);
     return 100;
@@ -774,6 +832,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   se.cnt = cnt;
   se.cand = cand;
   se.flag = flag;
+  se.nflag = nflag;
   se.nq = nq;
   se.D = (uint32_t)t.d;
   se.k = k;
@@ -781,6 +840,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   se.pmax = t.pmax;
   se.cutoff = qp.cutoff;
   se.max_abs = t.max_abs;
+  launch_zero_u32(nflag, 1, s);
   {
     KernelTimer tm(s);
     k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(se);
@@ -796,6 +856,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   ca.cnt = cnt;
   ca.cand = cand;
   ca.flag = flag;
+  ca.nflag = nflag;
   ca.out_d = trav.out_d;
   ca.out_i = trav.out_i;
   ca.nq = nq;
@@ -814,6 +875,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   KArgs fb = trav;
   fb.order = nullptr;
   fb.only = flag;
+  fb.only_count = nflag;
   const int rc = qp.screen_only ? 0 : launch_knn_lds(fb, M_L2, s);
   if (rc) {
     msg = "no fallback kernel";

@@ -58,7 +58,7 @@ def _torch():
 
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("launches", ctypes.c_uint64), ("ms", ctypes.c_double),
-                ("work", ctypes.c_double)]
+                ("work", ctypes.c_double), ("busy_ms", ctypes.c_double)]
 
 
 def _lib():
@@ -92,15 +92,17 @@ def timing_enable(on: bool = True) -> bool:
 
 
 def timing_read() -> dict:
-    """{kernel name: {"launches", "ms", "work"}} for the launches timed since the
-    last read (waits for their events)."""
+    """{kernel name: {"launches", "ms", "work", "busy_ms"}} for the launches
+    timed since the last read (waits for their events); busy_ms is the union of
+    the launches' intervals."""
     lib = _lib()
     cap = 256
     buf = (KernelTime * cap)()
     n = ctypes.c_uint32()
     _check(lib, lib.kdt_timing_read(buf, cap, ctypes.byref(n)))
     return {buf[j].name.decode(): {"launches": int(buf[j].launches), "ms": float(buf[j].ms),
-                                   "work": float(buf[j].work)} for j in range(min(n.value, cap))}
+                                   "work": float(buf[j].work), "busy_ms": float(buf[j].busy_ms)}
+            for j in range(min(n.value, cap))}
 
 
 def _check(lib, rc):

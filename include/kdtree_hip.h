@@ -50,7 +50,9 @@ enum kdt_error {
 
 /* Query flags. */
 #define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 leaf screen with a proven error margin,
-                                      FP64 recheck: identical results, fewer bytes.
+                                      FP64 recheck: identical results, fewer bytes
+                                      (the traversal's screen, ndim <= 16, for eps = 0
+                                      only; ignored for eps > 0).
                                       For 32 <= ndim <= 64, p = 2, eps = 0, k <= 32:
                                       matrix-core screen over every point, FP64
                                       recheck and a visit certificate; queries the

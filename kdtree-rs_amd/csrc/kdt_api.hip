@@ -447,9 +447,7 @@ int kdt_export(const kdt_tree* tree, double* data, uint32_t* indices, kdt_node_r
 void kdt_free(kdt_tree* tree) {
   if (!tree) return;
   kdt::Tree& t = tree->t;
-  void* ptrs[] = {t.nodes, t.boxes, t.pts, t.pts32, t.idx, t.rows, t.rows_bf, t.pnorm_bf};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
+  if (t.image) (void)hipFree(t.image);  // every array of the tree is a piece of it
   delete tree;
 }
 

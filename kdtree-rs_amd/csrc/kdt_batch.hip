@@ -92,6 +92,9 @@ __global__ __launch_bounds__(256, reg_min_waves<DT>()) void knn_batch_kernel(
   __shared__ uint32_t s_sm[GPB][STK];
   __shared__ double s_ax[GPB][DT];
   __shared__ double s_q[GPB][DT];
+  constexpr uint32_t CQ = SCREEN ? screen_ring<G>() : 1u;
+  __shared__ uint32_t s_cq[GPB][CQ];
+  __shared__ uint8_t s_cf[GPB][CQ];
   const uint32_t grp = threadIdx.x / G;
   const uint32_t lane = threadIdx.x & (G - 1);
   RegList<G> best;
@@ -107,7 +110,8 @@ __global__ __launch_bounds__(256, reg_min_waves<DT>()) void knn_batch_kernel(
     qi = g - items[it].q_begin;
     return true;
   };
-  knn_ww<DT, G, M, SCREEN>(cur, best, s_ax[grp], s_q[grp], s_sa[grp], s_sb[grp], s_sn[grp], s_sm[grp], next);
+  knn_ww<DT, G, M, SCREEN>(cur, best, s_ax[grp], s_q[grp], s_sa[grp], s_sb[grp], s_sn[grp], s_sm[grp], s_cq[grp],
+                           s_cf[grp], next);
 }
 
 template <int DT, int M, int STK>
@@ -251,7 +255,7 @@ int classify(const kdt_batch_item* items, uint64_t n_items, double p, double max
         screen_applies(t, qp)) {
       L.single.push_back(i);
     } else {
-      const bool scr = qp.fp32_screen && base.metric == M_L2;
+      const bool scr = qp.fp32_screen && base.metric == M_L2 && base.eps_factor == 1.0;  // the FP32 leaf screen: eps = 0
       cls[2 * w + (scr ? 1 : 0)].push_back(i);
     }
   }

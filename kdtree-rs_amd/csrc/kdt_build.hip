@@ -644,6 +644,27 @@ __global__ void k_order_by_box(DevNode* __restrict__ nodes, uint64_t n_nodes,
   if (shrank) nodes[id].b = nd.b | OBB_BIT;
 }
 
+// FP32 norms of the FP32 leaf screen (rows of up to 16 coordinates): pn32[r]
+// = fl32 of |v|^2 summed in FP64 from v = fl32(row r) (tree order), and
+// *pmax_bits = max over rows of |v| rounded up, as f32 bits.
+__global__ void k_norms32(const double* __restrict__ rows, uint64_t n, uint32_t D, float* __restrict__ pn32,
+                          uint32_t* pmax_bits) {
+  float mx = 0.0f;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (uint32_t c = 0; c < D; ++c) {
+      const double v = (double)(float)rows[r * D + c];
+      s += v * v;
+    }
+    pn32[r] = (float)s;
+    const double nr = sqrt(s);
+    float f = (float)nr;
+    if ((double)f < nr) f = nextafterf(f, INFINITY);
+    mx = fmaxf(mx, f);
+  }
+  atomicMax(pmax_bits, __float_as_uint(mx));  // non-negative floats order as integers
+}
+
 // Leaf buckets: rows of each leaf stored coordinate-major, f64 and f32.
 __global__ __launch_bounds__(64) void k_leaf_buckets(const double* __restrict__ rows, uint32_t D,
                                                      const uint32_t* __restrict__ leaves /*start,len*/,
@@ -740,13 +761,63 @@ struct ShapeCounter {
   }
 };
 
+// A device buffer: owned (alloc: its own hipMalloc, freed with it) or a view
+// of a piece of the tree image (place: freed with the image).
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  hipError_t alloc(size_t n) { return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)); }
+  bool owned = false;
+  ~DevBuf() { if (p && owned) (void)hipFree(p); }
+  hipError_t alloc(size_t n) {
+    owned = true;
+    return hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+  }
+  void place(char* image, size_t off) { p = reinterpret_cast<T*>(image + off); owned = false; }
   T* release() { T* q = p; p = nullptr; return q; }
 };
+
+// The tree image: every array a built tree keeps, in one device allocation,
+// each piece 256-byte aligned, in this order (sizes in elements):
+//   nodes    DevNode[n_nodes]
+//   boxes    f64[n_nodes][2][D]
+//   pts      f64[n][D]            leaf buckets, coordinate-major per leaf
+//   idx      u32[n]
+//   rows     f64[n][D]            tree order
+//   pts32    f32[n][D]            D <= 16: FP32 leaf buckets, same layout as pts
+//   pn32     f32[n]               D <= 16: fl32(|fl32(row)|^2), tree order
+//   rows_bf  bf16[n_pad][2][S]    8 <= D <= 64: matrix-core screen rows
+//   pnorm_bf u32[n_pad]           8 <= D <= 64: their split norms
+// (offsets: image_layout below; one hipMalloc and one hipFree per tree).
+struct ImageLayout {
+  size_t nodes, boxes, pts, idx, rows, pts32, pn32, rows_bf, pnorm_bf, total;
+};
+
+size_t image_piece(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~size_t(255); }
+
+ImageLayout image_layout(uint64_t n, uint64_t D, uint64_t n_nodes, bool narrow, bool wide, uint64_t n_pad,
+                         uint64_t stride) {
+  ImageLayout L{};
+  size_t o = 0;
+  auto put = [&](size_t& at, size_t bytes) {
+    at = o;
+    o += image_piece(bytes);
+  };
+  put(L.nodes, n_nodes * sizeof(DevNode));
+  put(L.boxes, n_nodes * 2 * D * 8);
+  put(L.pts, n * D * 8);
+  put(L.idx, n * 4);
+  put(L.rows, n * D * 8);
+  if (narrow) {
+    put(L.pts32, n * D * 4);
+    put(L.pn32, n * 4);
+  }
+  if (wide) {
+    put(L.rows_bf, n_pad * 2 * stride * 2);
+    put(L.pnorm_bf, n_pad * 4);
+  }
+  L.total = o;
+  return L;
+}
 
 }  // namespace
 
@@ -825,7 +896,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<LvlNode> d_lvl;
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
-  DevBuf<float> pts32;
+  DevBuf<float> pts32, pn32;
   DevBuf<uint32_t> pnorm_bf;
   DevBuf<uint16_t> rows_bf;
   DevBuf<uint32_t> idx, tidx, alist, blist, cnt, off, hist, node_len, d_leaves;
@@ -833,22 +904,36 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<Sel> sel;
   DevBuf<DevNode> nodes;
   DevBuf<int> err;
-  KDT_HIP_CHECK(d_lvl.alloc(tot_nodes));
-  KDT_HIP_CHECK(d_chunks.alloc(tot_chunks));
-  KDT_HIP_CHECK(rows.alloc(n * D));
-  KDT_HIP_CHECK(trows.alloc(n * D));
-  KDT_HIP_CHECK(part.alloc(max_chunks * 2 * D));
-  KDT_HIP_CHECK(boxes.alloc(t.n_nodes * 2 * D));
-  KDT_HIP_CHECK(pts.alloc(n * D));
-  if (D <= 16) KDT_HIP_CHECK(pts32.alloc(n * D));  // the FP32 screen of the compiled widths
+  const bool narrow = D <= 16;          // FP32 leaf buckets and norms: the traversal's FP32 screen
   const bool wide = D >= 8 && D <= 64;  // rows for the matrix-core screen
   const uint32_t stride32 = D <= 16 ? 16 : D <= 32 ? 32 : 64;
   const uint64_t n_pad = (n + 31) / 32 * 32;
-  if (wide) {
-    KDT_HIP_CHECK(rows_bf.alloc(n_pad * 2 * stride32));
-    KDT_HIP_CHECK(pnorm_bf.alloc(n_pad));
+  // The arrays the tree keeps: one device image (freed by kdt_free), the
+  // build's working arrays separately.
+  const ImageLayout IL = image_layout(n, D, t.n_nodes, narrow, wide, n_pad, stride32);
+  char* image = nullptr;
+  KDT_HIP_CHECK(hipMalloc((void**)&image, IL.total));
+  struct ImageGuard {
+    char*& p;
+    ~ImageGuard() { if (p) (void)hipFree(p); }
+  } image_guard{image};
+  nodes.place(image, IL.nodes);
+  boxes.place(image, IL.boxes);
+  pts.place(image, IL.pts);
+  idx.place(image, IL.idx);
+  rows.place(image, IL.rows);
+  if (narrow) {
+    pts32.place(image, IL.pts32);
+    pn32.place(image, IL.pn32);
   }
-  KDT_HIP_CHECK(idx.alloc(n));
+  if (wide) {
+    rows_bf.place(image, IL.rows_bf);
+    pnorm_bf.place(image, IL.pnorm_bf);
+  }
+  KDT_HIP_CHECK(d_lvl.alloc(tot_nodes));
+  KDT_HIP_CHECK(d_chunks.alloc(tot_chunks));
+  KDT_HIP_CHECK(trows.alloc(n * D));
+  KDT_HIP_CHECK(part.alloc(max_chunks * 2 * D));
   KDT_HIP_CHECK(tidx.alloc(n));
   KDT_HIP_CHECK(alist.alloc(n));
   KDT_HIP_CHECK(blist.alloc(n));
@@ -859,7 +944,6 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   KDT_HIP_CHECK(d_leaves.alloc(leaves.size()));
   KDT_HIP_CHECK(ukey.alloc(n));
   KDT_HIP_CHECK(sel.alloc(max_slots));
-  KDT_HIP_CHECK(nodes.alloc(t.n_nodes));
   KDT_HIP_CHECK(err.alloc(3));  // [0] error bits, [1] largest fix-up need of a level, [2] pmax bits
 
   KDT_HIP_CHECK(hipMemcpyAsync(d_lvl.p, all_nodes.data(), tot_nodes * sizeof(LvlNode), hipMemcpyHostToDevice, s));
@@ -937,6 +1021,10 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
     k_rows_split<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
         rows.p, n, Du, stride32, n_pad, rows_bf.p, pnorm_bf.p, reinterpret_cast<uint32_t*>(err.p) + 2);
   }
+  if (narrow) {
+    k_norms32<<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
+        rows.p, n, Du, pn32.p, reinterpret_cast<uint32_t*>(err.p) + 2);
+  }
   k_validate_nodes<<<(uint32_t)std::min<uint64_t>((t.n_nodes + 255) / 256, 4096), 256, 0, s>>>(
       nodes.p, t.n_nodes, n, Du, err.p);
   KDT_HIP_CHECK(hipGetLastError());
@@ -966,15 +1054,19 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   t.boxes = boxes.release();
   t.pts = pts.release();
   t.pts32 = pts32.release();
+  t.pn32 = pn32.release();
   t.idx = idx.release();
   t.rows = rows.release();
-  if (wide) {
+  t.rows_bf = rows_bf.release();
+  t.pnorm_bf = pnorm_bf.release();
+  if (wide || narrow) {
     float pm;
     std::memcpy(&pm, &h_pmax, 4);
     t.pmax = (double)pm * (1.0 + 1e-6);
-    t.rows_bf = rows_bf.release();
-    t.pnorm_bf = pnorm_bf.release();
   }
+  t.image = image;
+  t.image_bytes = IL.total;
+  image = nullptr;  // the tree owns it now
   return 0;
 }
 

@@ -15,6 +15,9 @@
 //                                      rows, as bf16 hi = bf16(v) then lo =
 //                                      bf16(v - hi), for the matrix-core screen
 //   pnorm_bf u32[n_pad]                fl32(|v|^2) (+inf on padding rows) as a packed bf16 pair (hi | lo << 16)
+//   pn32   f32[n]                      ndim <= 16: fl32(|v|^2) per tree position (the FP32 leaf screen)
+// All of them are pieces of one device allocation per tree, the tree image
+// (layout and order: image_layout in kdt_build.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,7 +58,10 @@ struct Tree {
   double max_abs = 0.0;  // max |coordinate| over the data (root box)
   uint16_t* rows_bf = nullptr;  // bf16 bit patterns
   uint32_t* pnorm_bf = nullptr;
-  double pmax = 0.0;     // upper bound on max |row32|
+  float* pn32 = nullptr;  // ndim <= 16: fl32(|row32|^2) per tree position, the FP32 leaf screen's norms
+  double pmax = 0.0;     // upper bound on max |row32| (ndim <= 64)
+  void* image = nullptr;  // the one device allocation every array above lives in (kdt_build.hip)
+  uint64_t image_bytes = 0;
 };
 
 // Device scratch of one call (leaf-order keys, screen lists, batch

@@ -107,10 +107,9 @@ int build_kargs(const Tree& t, const QueryParams& qp, const double* d_q, double*
   a.eps_factor = qp.eps_factor;
   a.p = qp.p;
   a.inv_p = 1.0 / qp.p;
-  const double u = std::ldexp(1.0, -24);
   const double D = (double)t.d;
-  a.screen_e = 4.0001 * std::sqrt(D) * u;
-  a.screen_gam = (1.0 + D * u / (1.0 - D * u)) * (1.0 + 1e-14);
+  a.pn32 = t.pn32;
+  a.pmax = t.pmax;
   a.screen_shrink = 1.0 + 2.0 * (D + 3.0) * std::ldexp(1.0, -53);
   a.max_abs = t.max_abs;
   a.stk = std::max<uint32_t>(2, 2 * t.depth);
@@ -160,8 +159,9 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
     // vs 15.7 K).
     if (t.d >= 32 && (G == 8 || (qp.group == 0 && G == 16))) G = qp.group == 0 ? 32 : 16;
     // The traversal's FP32 leaf screen reads the FP32 leaf buckets, which the
-    // build keeps for rows of up to 16 coordinates.
-    const bool screen = qp.fp32_screen && qp.metric == M_L2 && t.pts32 != nullptr;
+    // build keeps for rows of up to 16 coordinates, and defers its FP64
+    // rescoring, which leaves results unchanged only for eps = 0.
+    const bool screen = qp.fp32_screen && qp.metric == M_L2 && t.pts32 != nullptr && qp.eps_factor == 1.0;
     switch (qp.metric) {
       case M_L1: rc = launch_knn_l1(a, G, s); break;
       case M_L2: rc = launch_knn_l2(a, G, screen, s); break;

@@ -469,7 +469,7 @@ def run_serve(args):
         "metric": "k-NN queries/s + p50/p99 latency, served vs plain GPU, exact indices",
         "value": served_qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 screen (split bf16 matrix cores, f32 accumulate) + f64 recheck",
+        "dtype": "fp16 screen (matrix cores, f32 accumulate) + f64 recheck",
         "data": "synthetic (torch.randn float32 points, seed 5000+t; SplitMix64 Box-Muller queries)",
         "config": {"workload": "serve-32-trees-n1e5..8e6-d{16,64}-k10-batch1..65536-poisson-budget200GB",
                    "n_trees": len(specs), "n_points": int(sum(n for n, _ in specs)), "requests": args.requests,
@@ -664,7 +664,7 @@ def run_screen(args):
         "metric": "k-NN queries/s (exact indices), matrix-core screen, 64-d",
         "value": nq * args.steps / wall, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
-        "dtype": "split bf16 (matrix cores, f32 accumulate) + f64 recheck",
+        "dtype": "fp16 screen (matrix cores, f32 accumulate) + f64 recheck",
         "data": "synthetic (torch.randn float32, seed 64)",
         "config": {"workload": "knn-normal-n1048576-d64-q65536-k10-leafsize32-p2"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -54,8 +54,10 @@ enum kdt_error {
                                       (the traversal's screen, ndim <= 16, for eps = 0
                                       only; ignored for eps > 0).
                                       For 32 <= ndim <= 64, p = 2, eps = 0, k <= 32:
-                                      matrix-core screen over every point, FP64
-                                      recheck and a visit certificate; queries the
+                                      matrix-core screen over every point (rows
+                                      rounded to fp16 in a per-tree frame, with a
+                                      proven error bound), FP64 recheck and a visit
+                                      certificate; queries the
                                       certificate does not cover run the traversal */
 #define KDT_QUERY_ASSUME_FINITE 0x2u /* kdt_query_device only: the caller has already
                                         checked that every query coordinate is finite

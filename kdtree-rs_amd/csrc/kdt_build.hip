@@ -701,40 +701,59 @@ __global__ void k_validate_nodes(const DevNode* __restrict__ nodes, uint64_t n_n
   if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(err, 8);
 }
 
-// Matrix-core screen rows: tree-order rows rounded to f32 (v), zero-padded to
-// `stride` columns and n_pad rows, each stored as two bf16 halves,
-// hi = bf16(v) and lo = bf16(v - hi): row r is [hi x stride | lo x stride] in
-// rows_bf.  pnorm_bf = pn = fl32(|v|^2) (+inf on padding rows) split the same
-// way, packed as bf16(pn) | bf16(pn - hi) << 16 (the residual 0 for +inf): the
-// word the screen's norm block takes as its operand.  *pmax_bits = max over
-// rows of |v| rounded up, as f32 bits.
-__global__ void k_rows_split(const double* __restrict__ rows, uint64_t n, uint32_t D, uint32_t stride,
-                             uint64_t n_pad, uint16_t* __restrict__ rows_bf, uint32_t* __restrict__ pnorm_bf,
-                             uint32_t* pmax_bits) {
+// The matrix-core screen's frame, from the root box (one wave): centre c_j =
+// lo_j / 2 + hi_j / 2 and sigma = 2^s with M sigma < 2^8, M = max_j
+// max(|fl64(hi_j - c_j)|, |fl64(lo_j - c_j)|) (every row's fl64(x_j - c_j) lies
+// between the two, rounding being monotonic), s = 8 - (ilogb(M) + 1), at most
+// 1000 (M = 0: s = 0).  frame = [c_0 .. c_{D-1}, sigma].
+__global__ void k_screen_frame(const double* __restrict__ boxes, uint32_t D, double* __restrict__ frame) {
+  const uint32_t lane = threadIdx.x;
+  double m = 0.0;
+  for (uint32_t j = lane; j < D; j += 64) {
+    const double lo = boxes[j], hi = boxes[D + j];
+    const double c = 0.5 * lo + 0.5 * hi;
+    frame[j] = c;
+    m = fmax(m, fmax(fabs(hi - c), fabs(lo - c)));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if (lane == 0) {
+    const int s = m > 0.0 ? min(8 - (ilogb(m) + 1), 1000) : 0;
+    frame[D] = ldexp(1.0, s);
+  }
+}
+
+// Matrix-core screen rows: tree-order rows x as fp16 of fl32(sigma fl64(x -
+// c)) (screen_f16), zero-padded to `stride` columns and n_pad rows, in rows_h;
+// pnorm_h = the norm-block words of pn = fl32(|row|^2) (the fp16 values
+// squared and summed in FP64; +inf on padding rows); *pc_bits = max over rows
+// of |x - c| (FP64), rounded up, as f32 bits.
+__global__ void k_rows_h(const double* __restrict__ rows, uint64_t n, uint32_t D, uint32_t stride, uint64_t n_pad,
+                         const double* __restrict__ frame, uint16_t* __restrict__ rows_h, uint2* __restrict__ pnorm_h,
+                         uint32_t* pc_bits) {
+  const double sigma = frame[D];
   float mx = 0.0f;
   for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_pad;
        r += (uint64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
+    double s = 0.0, pc = 0.0;
     for (uint32_t c = 0; c < stride; ++c) {
-      const float v = (r < n && c < D) ? (float)rows[r * D + c] : 0.0f;
-      const __bf16 hi = (__bf16)v;
-      const __bf16 lo = (__bf16)(v - (float)hi);
-      rows_bf[r * 2 * stride + c] = __builtin_bit_cast(uint16_t, hi);
-      rows_bf[r * 2 * stride + stride + c] = __builtin_bit_cast(uint16_t, lo);
-      s += (double)v * (double)v;
+      const double y = (r < n && c < D) ? rows[r * D + c] - frame[c] : 0.0;
+      const _Float16 h = screen_f16(y, sigma);
+      rows_h[r * stride + c] = __builtin_bit_cast(uint16_t, h);
+      s += (double)h * (double)h;
+      pc += y * y;
     }
-    const float pn = r < n ? (float)s : INFINITY;
-    const __bf16 ph = (__bf16)pn;
-    const __bf16 pl = (__bf16)(__builtin_isinf(pn) ? 0.0f : pn - (float)ph);
-    pnorm_bf[r] = (uint32_t)__builtin_bit_cast(uint16_t, ph) | ((uint32_t)__builtin_bit_cast(uint16_t, pl) << 16);
+    pnorm_h[r] = screen_norm_words(r < n ? (float)s : INFINITY);
     if (r < n) {
-      const double nr = sqrt(s);
+      // |fl64(x_j - c_j)| >= |x_j - c_j| / (1 + 2^-53); the FP64 sum and root
+      // add at most (D + 2) 2^-53 more.
+      const double nr = sqrt(pc) * (1.0 + 0x1p-44);
       float f = (float)nr;
       if ((double)f < nr) f = nextafterf(f, INFINITY);
       mx = fmaxf(mx, f);
     }
   }
-  atomicMax(pmax_bits, __float_as_uint(mx));  // non-negative floats order as integers
+  atomicMax(pc_bits, __float_as_uint(mx));  // non-negative floats order as integers
 }
 
 __global__ void k_original_order(const double* __restrict__ rows, const uint32_t* __restrict__ idx,
@@ -785,11 +804,12 @@ struct DevBuf {
 //   rows     f64[n][D]            tree order
 //   pts32    f32[n][D]            D <= 16: FP32 leaf buckets, same layout as pts
 //   pn32     f32[n]               D <= 16: fl32(|fl32(row)|^2), tree order
-//   rows_bf  bf16[n_pad][2][S]    8 <= D <= 64: matrix-core screen rows
-//   pnorm_bf u32[n_pad]           8 <= D <= 64: their split norms
+//   frame    f64[D + 1]           8 <= D <= 64: matrix-core screen frame (centre, sigma)
+//   rows_h   f16[n_pad][S]        8 <= D <= 64: matrix-core screen rows
+//   pnorm_h  u32x2[n_pad]         8 <= D <= 64: their norms as norm-block words
 // (offsets: image_layout below; one hipMalloc and one hipFree per tree).
 struct ImageLayout {
-  size_t nodes, boxes, pts, idx, rows, pts32, pn32, rows_bf, pnorm_bf, total;
+  size_t nodes, boxes, pts, idx, rows, pts32, pn32, frame, rows_h, pnorm_h, total;
 };
 
 size_t image_piece(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~size_t(255); }
@@ -812,8 +832,9 @@ ImageLayout image_layout(uint64_t n, uint64_t D, uint64_t n_nodes, bool narrow, 
     put(L.pn32, n * 4);
   }
   if (wide) {
-    put(L.rows_bf, n_pad * 2 * stride * 2);
-    put(L.pnorm_bf, n_pad * 4);
+    put(L.frame, (D + 1) * 8);
+    put(L.rows_h, n_pad * stride * 2);
+    put(L.pnorm_h, n_pad * 8);
   }
   L.total = o;
   return L;
@@ -897,8 +918,9 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
   DevBuf<float> pts32, pn32;
-  DevBuf<uint32_t> pnorm_bf;
-  DevBuf<uint16_t> rows_bf;
+  DevBuf<double> frame;
+  DevBuf<uint2> pnorm_h;
+  DevBuf<uint16_t> rows_h;
   DevBuf<uint32_t> idx, tidx, alist, blist, cnt, off, hist, node_len, d_leaves;
   DevBuf<uint64_t> ukey;
   DevBuf<Sel> sel;
@@ -927,8 +949,9 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
     pn32.place(image, IL.pn32);
   }
   if (wide) {
-    rows_bf.place(image, IL.rows_bf);
-    pnorm_bf.place(image, IL.pnorm_bf);
+    frame.place(image, IL.frame);
+    rows_h.place(image, IL.rows_h);
+    pnorm_h.place(image, IL.pnorm_h);
   }
   KDT_HIP_CHECK(d_lvl.alloc(tot_nodes));
   KDT_HIP_CHECK(d_chunks.alloc(tot_chunks));
@@ -944,14 +967,16 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   KDT_HIP_CHECK(d_leaves.alloc(leaves.size()));
   KDT_HIP_CHECK(ukey.alloc(n));
   KDT_HIP_CHECK(sel.alloc(max_slots));
-  KDT_HIP_CHECK(err.alloc(3));  // [0] error bits, [1] largest fix-up need of a level, [2] pmax bits
+  // [0] error bits, [1] largest fix-up need of a level, [2] pmax bits (FP32
+  // leaf screen), [3] max |x - c| bits (matrix-core screen)
+  KDT_HIP_CHECK(err.alloc(4));
 
   KDT_HIP_CHECK(hipMemcpyAsync(d_lvl.p, all_nodes.data(), tot_nodes * sizeof(LvlNode), hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(d_chunks.p, all_chunks.data(), tot_chunks * sizeof(Chunk), hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(d_leaves.p, leaves.data(), leaves.size() * 4, hipMemcpyHostToDevice, s));
   KDT_HIP_CHECK(hipMemcpyAsync(rows.p, d_data, n * D * 8, hipMemcpyDeviceToDevice, s));
   KDT_HIP_CHECK(hipMemsetAsync(hist.p, 0, max_slots * 256 * 4, s));
-  KDT_HIP_CHECK(hipMemsetAsync(err.p, 0, 12, s));
+  KDT_HIP_CHECK(hipMemsetAsync(err.p, 0, 16, s));
 
   // Finiteness of the input (the root bbox check, src/build.rs:51-57).
   launch_check_finite(rows.p, n * D, err.p, s);
@@ -1018,8 +1043,9 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
                                                                     node_len.p);
   k_leaf_buckets<<<(uint32_t)(leaves.size() / 2), 64, 0, s>>>(rows.p, Du, d_leaves.p, pts.p, pts32.p);
   if (wide) {
-    k_rows_split<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
-        rows.p, n, Du, stride32, n_pad, rows_bf.p, pnorm_bf.p, reinterpret_cast<uint32_t*>(err.p) + 2);
+    k_screen_frame<<<1, 64, 0, s>>>(boxes.p, Du, frame.p);
+    k_rows_h<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
+        rows.p, n, Du, stride32, n_pad, frame.p, rows_h.p, pnorm_h.p, reinterpret_cast<uint32_t*>(err.p) + 3);
   }
   if (narrow) {
     k_norms32<<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
@@ -1032,8 +1058,10 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   std::vector<double> rb(2 * D);
   KDT_HIP_CHECK(hipMemcpyAsync(rb.data(), boxes.p, 2 * D * 8, hipMemcpyDeviceToHost, s));
   KDT_HIP_CHECK(hipMemcpyAsync(&h_err, err.p, 4, hipMemcpyDeviceToHost, s));
-  uint32_t h_pmax = 0;
-  KDT_HIP_CHECK(hipMemcpyAsync(&h_pmax, err.p + 2, 4, hipMemcpyDeviceToHost, s));
+  uint32_t h_pmax[2] = {0, 0};
+  KDT_HIP_CHECK(hipMemcpyAsync(h_pmax, err.p + 2, 8, hipMemcpyDeviceToHost, s));
+  double h_sigma = 1.0;
+  if (wide) KDT_HIP_CHECK(hipMemcpyAsync(&h_sigma, frame.p + D, 8, hipMemcpyDeviceToHost, s));
   KDT_HIP_CHECK(hipStreamSynchronize(s));
   if (h_err & 8) {
     msg = "internal error: malformed node after the build";
@@ -1057,12 +1085,19 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   t.pn32 = pn32.release();
   t.idx = idx.release();
   t.rows = rows.release();
-  t.rows_bf = rows_bf.release();
-  t.pnorm_bf = pnorm_bf.release();
-  if (wide || narrow) {
+  t.frame = frame.release();
+  t.rows_h = rows_h.release();
+  t.pnorm_h = pnorm_h.release();
+  if (narrow) {
     float pm;
-    std::memcpy(&pm, &h_pmax, 4);
+    std::memcpy(&pm, &h_pmax[0], 4);
     t.pmax = (double)pm * (1.0 + 1e-6);
+  }
+  if (wide) {
+    float pc;
+    std::memcpy(&pc, &h_pmax[1], 4);
+    t.scr_pc = (double)pc;  // rounded up in k_rows_h
+    t.scr_sigma = h_sigma;
   }
   t.image = image;
   t.image_bytes = IL.total;

@@ -9,12 +9,19 @@
 //   idx    u32[n_points]               original row of tree position
 //   rows   f64[n_points][ndim]         rows in tree order (row-major), the
 //                                      reference's Tree::data
-//   rows_bf bf16[n_pad][2][stride]     8 <= ndim <= 64 only: rows in tree order
-//                                      rounded to f32 (v), zero-padded to stride
-//                                      16, 32 or 64 and to n_pad = 32 * ceil(n / 32)
-//                                      rows, as bf16 hi = bf16(v) then lo =
-//                                      bf16(v - hi), for the matrix-core screen
-//   pnorm_bf u32[n_pad]                fl32(|v|^2) (+inf on padding rows) as a packed bf16 pair (hi | lo << 16)
+//   frame  f64[ndim + 1]               8 <= ndim <= 64 only: the matrix-core
+//                                      screen's frame, centre c (middle of the
+//                                      root box per coordinate), then the scale
+//                                      sigma = 2^s that maps the rows' largest
+//                                      |x - c| below 2^8
+//   rows_h f16[n_pad][stride]          8 <= ndim <= 64 only: rows in tree order as
+//                                      fp16(fl32(sigma (x - c))) (fp16 subnormals
+//                                      flushed to zero), zero-padded to stride 16,
+//                                      32 or 64 and to n_pad = 32 * ceil(n / 32) rows
+//   pnorm_h u32x2[n_pad]               their squared norms pn = fl32(|row_h|^2)
+//                                      (+inf on padding rows) as three bf16
+//                                      pieces, packed as the screen's norm-block
+//                                      operand words (1 | p1 << 16, p2 | p3 << 16)
 //   pn32   f32[n]                      ndim <= 16: fl32(|v|^2) per tree position (the FP32 leaf screen)
 // All of them are pieces of one device allocation per tree, the tree image
 // (layout and order: image_layout in kdt_build.hip).
@@ -56,10 +63,13 @@ struct Tree {
   uint32_t* idx = nullptr;
   double* rows = nullptr;
   double max_abs = 0.0;  // max |coordinate| over the data (root box)
-  uint16_t* rows_bf = nullptr;  // bf16 bit patterns
-  uint32_t* pnorm_bf = nullptr;
+  double* frame = nullptr;     // matrix-core screen: centre[ndim], sigma
+  uint16_t* rows_h = nullptr;  // matrix-core screen rows, fp16 bit patterns
+  uint2* pnorm_h = nullptr;    // their norms, norm-block operand words
+  double scr_sigma = 1.0;      // the frame's sigma (host copy)
+  double scr_pc = 0.0;         // upper bound on max over rows of |x - c| (FP64 rows)
   float* pn32 = nullptr;  // ndim <= 16: fl32(|row32|^2) per tree position, the FP32 leaf screen's norms
-  double pmax = 0.0;     // upper bound on max |row32| (ndim <= 64)
+  double pmax = 0.0;     // ndim <= 16: upper bound on max |row32|
   void* image = nullptr;  // the one device allocation every array above lives in (kdt_build.hip)
   uint64_t image_bytes = 0;
 };
@@ -190,6 +200,34 @@ inline int requested_group(uint32_t flags) {
 // Metric, cutoff and eps factor from (p, max_distance, eps) with the checks
 // and messages of src/query.rs:49-62 and src/metric.rs:45-71; 0 or an error code.
 int query_params(double p, double max_distance, double eps, QueryParams& qp, std::string& msg);
+
+// The matrix-core screen's rounding of one coordinate already moved to the
+// tree's frame (y = fl64(x - c), sigma the frame's scale): fp16 of
+// fl32(sigma y), round to nearest even, with fp16 subnormals flushed to zero
+// (the matrix cores then never see one) and the value clamped to +-2^11 first
+// (tree rows stay below 2^8 by the choice of sigma; a query beyond 2^11 is
+// not screened, the clamp only keeps its operands finite).  Per coordinate
+// |sigma y - result| <= 2^-11 (1 + 2^-12) |sigma y| + 2^-14 within the clamp
+// (2^-24 for the FP32 step, 2^-11 for fp16, 2^-14 for what is flushed).
+__device__ __forceinline__ _Float16 screen_f16(double y, double sigma) {
+  const float v = fminf(fmaxf((float)(y * sigma), -2048.0f), 2048.0f);
+  const _Float16 h = (_Float16)v;
+  return fabsf((float)h) < 0x1p-14f ? (_Float16)0.0f : h;
+}
+
+// The norm-block words of a squared norm pn (FP32): pn as three bf16 pieces
+// p1 = bf16(pn), p2 = bf16(pn - p1), p3 = bf16(pn - p1 - p2) (each difference
+// exact in FP32; p1 + p2 + p3 = pn up to 2^-27 pn), packed as
+// (1 | p1 << 16, p2 | p3 << 16): the bf16 operand (1, p1, p2, p3) of the
+// screen's norm block.  pn = +inf gives (inf, 0, 0).
+__device__ __forceinline__ uint2 screen_norm_words(float pn) {
+  const __bf16 p1 = (__bf16)pn;
+  const float r1 = __builtin_isinf(pn) ? 0.0f : pn - (float)p1;
+  const __bf16 p2 = (__bf16)r1;
+  const __bf16 p3 = (__bf16)(r1 - (float)p2);
+  return make_uint2(0x3F80u | ((uint32_t)__builtin_bit_cast(uint16_t, p1) << 16),
+                    (uint32_t)__builtin_bit_cast(uint16_t, p2) | ((uint32_t)__builtin_bit_cast(uint16_t, p3) << 16));
+}
 
 }  // namespace kdt
 struct kdt_tree;

@@ -11,11 +11,11 @@
 //
 // Steps for a query matrix against one tree:
 //   1. k_screen: for a tile of 32 queries and a slice of the points, the
-//      matrix cores form  A = |q|^2 + |p|^2 - 2 q.p  from bf16 halves of the
-//      FP32-rounded rows and norms (three v_mfma_f32_32x32x16_bf16 per 16
-//      coordinates for -2 q.p, one more that adds the two norms) and each lane
-//      keeps the KP = 16 smallest A of the points it sees, with the points'
-//      tree positions;
+//      matrix cores form  A = |q^|^2 + |p^|^2 - 2 q^.p^  from the rows rounded
+//      to fp16 in the tree's frame (one v_mfma_f32_32x32x16_f16 per 16
+//      coordinates for -2 q^.p^, one v_mfma_f32_32x32x16_bf16 more that adds
+//      the two norms) and each lane keeps the KP = 16 smallest A of the points
+//      it sees, with the points' tree positions;
 //   2. k_screen_select: the k-th smallest A over all of a query's lists gives
 //      A_k, and from it a threshold tau_A such that every point whose FP64
 //      distance could be among the k best has A <= tau_A (error bound
@@ -34,26 +34,33 @@
 //      this top-k.  Queries whose certificate fails are flagged;
 //   4. the traversal kernels run the flagged queries (KArgs::only).
 //
-// Error bound (u = 2^-24, q~ and p~ the rows rounded to FP32, P the largest
-// |p~| of the tree, N = |q~| + P).  Splitting an FP32 x into bf16 halves
-// (precision 8 bits) leaves |x - xh - xl| <= 2^-16 |x| and |xl| <= 2^-8 |x|
-// (1 + 2^-8), so the three products miss at most 3.1 2^-16 |x| |y| per
-// coordinate (x = -2 q~_i, exact) and the split norms qn = fl32(|q~|^2),
-// pn = fl32(|p~|^2) at most 2^-16 (qn + pn); the matrix core adds the 3 D + 4
-// exact terms in FP32, whose magnitudes sum to at most (1 + 2^-6) N^2:
-//   |A - |q~ - p~|^2| <= (1.02 gamma + 3.1 2^-16 + 1.01 u) N^2 + flushes
-//     <= E2 = (2 g + 8u) N^2 + D 2^-110 (1 + N)^2,
-//     gamma = (3D+3)u / (1 - (3D+3)u),  g = 3.1 2^-16 + 1.03 (3D+4)u / (1 - (3D+4)u)
-//     (the split residuals, FP32 rounding of the norms, the accumulation;
-//      flushed subnormal halves and products);
-//   | |q~ - p~| - |q - p| | <= e1 = u (1 + 2u) N + sqrt(D) 2^-148;
-//   the FP64 reduced distance d of the reference satisfies d / s <= |q-p|^2
-//     <= d s with s = 1 + 2 (D + 3) 2^-53.
-// So U(A) = s (sqrt(A + E2) + e1)^2 bounds d from above, and a point with
-// d <= tau has A <= tau_A = (sqrt(s tau) + e1)^2 + E2.  With tau =
-// min(U(A_k), cutoff): at least k points have d <= U(A_k), so the k-th best
-// distance is <= tau and every point of the top-k is a candidate.
-// Rows with a coordinate above 1e18 in magnitude are not screened (flagged).
+// Error bound.  The screen works in a frame fixed per tree at the build:
+// centre c (the middle of the root box) and a power of two sigma with
+// sigma |fl64(x_j - c_j)| < 2^8 for every row.  A row x becomes
+// x^_j = screen_f16(fl64(x_j - c_j), sigma) (fp16 of the FP32 value, round to
+// nearest even, fp16 subnormals flushed), so per coordinate
+//   |sigma (x_j - c_j) - x^_j| <= a sigma |x_j - c_j| + b,
+//   a = 2^-11 (1 + 2^-12),  b = 2^-14 (1 + 2^-60)
+// (the FP64 difference adds 2^-53 relative), and with R = sigma |q - p|
+// (both rows moved by the same c) the triangle inequality gives
+//   | |q^ - p^| - R | <= e1 = a sigma (|q - c| + P_c) + 2 b sqrt(D),
+// P_c >= |p - c| for every row p of the tree.  The matrix cores sum in FP32
+// the D exact products (-2 q^_j) p^_j (an fp16 product is exact in FP32) and
+// the norms qn = fl32(|q^|^2), pn = fl32(|p^|^2) as three bf16 pieces each
+// (sum = the norm up to 2^-27 of it); with any order and any rounding of at
+// most 2^-23 per addition (D + 5 additions, the pieces' magnitudes at most
+// (1 + 2^-8) times their norm),
+//   |A - |q^ - p^|^2| <= E2 = ((1 + 2^-7) gamma + 1.13 2^-24) Nh^2 + 2^-118,
+//   gamma = (D + 5) 2^-23 / (1 - (D + 5) 2^-23),
+//   Nh = sigma (|q - c| + P_c) (1 + a) + 2 b sqrt(D)  >=  |q^| + |p^|.
+// The FP64 reduced distance d of the reference satisfies d / s <= |q - p|^2
+// <= d s with s = 1 + 2 (D + 3) 2^-53.  So U(A) = s (sqrt(A + E2) + e1)^2 /
+// sigma^2 bounds d from above, and a point with d <= tau has A <= tau_A =
+// (sigma sqrt(s tau) + e1)^2 + E2.  With tau = min(U(A_k), cutoff): at least
+// k points have d <= U(A_k), so the k-th best distance is <= tau and every
+// point of the top-k is a candidate.  A query with a coordinate of
+// sigma |fl64(q_j - c_j)| above 2^11 is not screened (flagged): below that
+// its fp16 values, -2 q^ and |q^|^2 stay in range.
 
 #include <algorithm>
 #include <cmath>
@@ -69,6 +76,7 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 constexpr uint32_t CAND_CAP = 64;  // candidates per query (one per lane of the recheck)
 constexpr int KP = 16;             // entries per lane list
@@ -80,8 +88,9 @@ constexpr int LISTS_PER_LANE = 16;
 constexpr uint32_t MAX_SLICES = 32 * LISTS_PER_LANE;
 
 struct SArgs {
-  const uint16_t* __restrict__ rows_bf;  // [n_pad][hi 2 KS | lo 2 KS] bf16, tree order, zero-padded
-  const uint32_t* __restrict__ pnorm_bf;  // [n_pad] |p~|^2 (+inf on padding rows), packed bf16 hi | lo << 16
+  const uint16_t* __restrict__ rows_h;  // [n_pad][2 KS] fp16, tree order, zero-padded
+  const uint2* __restrict__ pnorm_h;    // [n_pad] their norms, norm-block words (+inf on padding rows)
+  const double* __restrict__ frame;     // [D + 1] centre, sigma
   const double* __restrict__ q;      // [nq][D]
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
   uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
@@ -117,14 +126,14 @@ constexpr int screen_waves() { return KS <= 16 ? 4 : 8; }
 template <int KS>
 constexpr int screen_tiles_per_step() { return KS <= 8 ? KDT_SCREEN_TS8 : KS <= 16 ? KDT_SCREEN_TS16 : KDT_SCREEN_TS32; }
 
-// One step: TS point tiles, each 32 rows of 4 KS bf16 ([hi 2 KS | lo 2 KS],
-// KS / 2 uint4), staged global -> registers -> LDS by the whole workgroup of
-// SW = screen_waves waves: thread u copies uint4 u, u + 64 SW of the step
-// (coalesced within each tile), and threads 0 .. 32 TS - 1 the split |p~|^2.
+// One step: TS point tiles, each 32 rows of 2 KS fp16 (KS / 4 uint4), staged
+// global -> registers -> LDS by the whole workgroup of SW = screen_waves
+// waves: thread u copies uint4 u, u + 64 SW of the step (coalesced within each
+// tile), and threads 0 .. 32 TS - 1 the norm words.
 // (Nothing is read past a tile: TU, SU and the clamps bound every index.)
 template <int KS>
 struct Stage {
-  static constexpr int RU = KS / 2;                          // uint4 per row
+  static constexpr int RU = KS / 4;                          // uint4 per row
   static constexpr int RS = RU + 1;                          // LDS row stride (one uint4 of padding)
   static constexpr int TU = 32 * RU;                         // uint4 per tile
   static constexpr int TS = screen_tiles_per_step<KS>();
@@ -143,9 +152,9 @@ struct Stage {
 // re-read), and local tile i is global tile tb + i ts.
 template <int KS>
 __device__ __forceinline__ void stage_load(const SArgs& a, uint32_t lt0, uint32_t t1, uint32_t tb, uint32_t ts,
-                                           uint32_t tid, uint4& v0, uint4& v1, uint32_t& pn) {
+                                           uint32_t tid, uint4& v0, uint4& v1, uint2& pn) {
   using St = Stage<KS>;
-  const uint4* src = reinterpret_cast<const uint4*>(a.rows_bf);
+  const uint4* src = reinterpret_cast<const uint4*>(a.rows_h);
   auto gtile = [&](uint32_t i) -> size_t { return (size_t)tb + (size_t)min(lt0 + i, t1 - 1) * ts; };
   {
     const uint32_t u = min(tid, (uint32_t)St::SU - 1);
@@ -156,15 +165,15 @@ __device__ __forceinline__ void stage_load(const SArgs& a, uint32_t lt0, uint32_
     v1 = src[gtile(u / St::TU) * St::TU + u % St::TU];
   }
   const uint32_t pi = min(tid, (uint32_t)St::TS * 32 - 1);
-  pn = a.pnorm_bf[gtile(pi / 32) * 32 + pi % 32];
+  pn = a.pnorm_h[gtile(pi / 32) * 32 + pi % 32];
 }
 
 // LDS rows are padded by one uint4 (stride KS / 2 + 1 uint4), so the 32 lanes
 // of a half-wave reading 32 different rows hit distinct banks; tile i of the
 // step starts at rows + i 32 RS and pn + i 32.
 template <int KS>
-__device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, uint32_t pnv, uint32_t tid,
-                                            uint4* rows, uint32_t* pn) {
+__device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, const uint2& pnv, uint32_t tid,
+                                            uint4* rows, uint2* pn) {
   using St = Stage<KS>;
   auto at = [](uint32_t u) { return (u / St::TU) * 32 * St::RS + ((u % St::TU) / St::RU) * St::RS + u % St::RU; };
   if (tid < (uint32_t)St::SU) rows[at(tid)] = v0;
@@ -177,14 +186,14 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, ui
 // (A operand, rows); the SW waves of a workgroup take SW consecutive query
 // tiles and share each point tile through LDS (one HBM read per workgroup
 // instead of one per wave).  Lane l holds query j = l & 31, coordinates
-// [h KS, h KS + KS) with h = l >> 5, in both operands; instruction s of a
-// product contracts coordinates 8 s .. 8 s + 7 of each half.  Each FP32 value
-// v (of -2 q~ and of p~) is split into bf16 halves hi + lo (lo = bf16(v - hi))
-// and -2 q.p is formed by three v_mfma_f32_32x32x16_bf16 per 16 coordinates:
-// qh.ph + ql.ph + qh.pl (bf16 x bf16 products are exact in FP32; the dropped
-// ql.pl and the split residual are in the error bound); one more adds the
-// split norms.  Element r of the accumulator is then A for query j and point
-// row i = 8 (r / 4) + 4 h + (r % 4), r = 0..15.
+// [h KS, h KS + KS) with h = l >> 5, in both operands; instruction v of the
+// product contracts coordinates 8 v .. 8 v + 7 of each half.  The query
+// operand is -2 q^ (exact in fp16), so the NS = KS / 8 v_mfma_f32_32x32x16_f16
+// sum -2 q^.p^ in FP32; one v_mfma_f32_32x32x16_bf16 more adds the norms
+// (lanes of half 0: query side (q1, q2, q3, 1, 1, 1, 0, 0), point side
+// (1, 1, 1, p1, p2, p3, 0, 0), the norms' bf16 pieces).  Element r of the
+// accumulator is then A for query j and point row i = 8 (r / 4) + 4 h +
+// (r % 4), r = 0..15.
 //
 // A value under the lane's threshold is not inserted at once: it is appended
 // to the lane's buffer of IB slots in LDS, and after every CHK elements, if any
@@ -206,11 +215,11 @@ static_assert(KDT_SCREEN_IB8 > CHK, "a buffer holds more than one check interval
 
 template <int KS>
 struct Screen {
-  static constexpr int NS = KS / 8;  // bf16x8 operands per lane half
-  static constexpr int RU = KS / 2, RS = RU + 1;
+  static constexpr int NS = KS / 8;  // f16x8 operands per lane half
+  static constexpr int RU = KS / 4, RS = RU + 1;
   static constexpr uint32_t IB = insert_slots<KS>();
-  bf16x8 qh[NS], ql[NS];  // -2 q~, split
-  bf16x8 qnb;             // norm block, B side: (qn_hi, qn_lo, 1, 1, 0...) in half 0, zero in half 1
+  f16x8 qh[NS];     // -2 q^
+  bf16x8 qnb;       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
   float la[KP];     // the KP smallest A seen by this lane, ascending
   uint32_t li[KP];  // their tree positions
   float thr;        // la[KP - 1] at the last merge
@@ -247,33 +256,21 @@ struct Screen {
     cnt = 0;
   }
 
-  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint32_t* pns) {
+  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
     const uint4* rb = rows + j * RS + h * NS;
-    // Norm block, A side: (1, 1, pn_hi, pn_lo, 0...) of row j in half 0, so one
-    // more instruction adds qn + pn (four exact terms) to every element and the
-    // accumulator ends as A itself.  Padding rows: pn = +inf, pn_lo = 0, A = +inf.
-    const uint32_t pw = pns[j];  // split in the build
-    const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u);
-    // The point operands of 16 coordinates are read one group ahead of their
-    // three products (left to itself the scheduler reads all 2 NS operands
-    // first, and the extra registers push the query halves to scratch).
+    // Norm block, point side, of row j in half 0 (built with the rows).
+    // Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
+    const uint2 pw = pns[j];
+    const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw.x, pw.y, 0u) : make_uint4(0u, 0u, 0u, 0u);
+    // The point operands are read one group ahead of their product.
     floatx16 acc = {};
-    uint4 ch = rb[0], cl = rb[2 * NS];
+    uint4 cur = rb[0];
 #pragma unroll
     for (int v = 0; v < NS; ++v) {
-      uint4 nh = ch, nl = cl;
-      if (v + 1 < NS) {
-        nh = rb[v + 1];
-        nl = rb[2 * NS + v + 1];
-      }
-      const bf16x8 ph = __builtin_bit_cast(bf16x8, ch);
-      const bf16x8 pl = __builtin_bit_cast(bf16x8, cl);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, qh[v], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ph, ql[v], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(pl, qh[v], acc, 0, 0, 0);
-      __builtin_amdgcn_sched_barrier(0);
-      ch = nh;
-      cl = nl;
+      uint4 nxt = cur;
+      if (v + 1 < NS) nxt = rb[v + 1];
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, cur), qh[v], acc, 0, 0, 0);
+      cur = nxt;
     }
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb, acc, 0, 0, 0);
     // Element r is row rowbase | (8 (r / 4) + r % 4): disjoint bits, one
@@ -305,7 +302,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int RS = Stage<KS>::RS;
   constexpr int TS = Stage<KS>::TS;
   __shared__ uint4 s_rows[2][TS * 32 * RS];
-  __shared__ __align__(16) uint32_t s_pn[2][TS * 32];
+  __shared__ uint2 s_pn[2][TS * 32];
   __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
@@ -314,27 +311,26 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   const uint64_t qrow = qtile * 32 + j;
   const bool qvalid = qrow < a.nq;
   Screen<KS> sc;
+  const double sigma = a.frame[a.D];
   double qn_part = 0.0;
 #pragma unroll
   for (int s = 0; s < KS; ++s) {
     // Loads at clamped (in-bounds) indices, all issued before any is used;
     // coordinates past D and rows past nq are zero.
     const uint32_t c = h * KS + s;
-    const double raw = a.q[min(qrow, a.nq - 1) * a.D + min(c, a.D - 1)];
-    const float v = (qvalid && c < a.D) ? (float)raw : 0.0f;
-    const float w = -2.0f * v;  // exact
-    const __bf16 hi = (__bf16)w;
-    sc.qh[s / 8][s % 8] = hi;
-    sc.ql[s / 8][s % 8] = (__bf16)(w - (float)hi);
+    const uint32_t cc = min(c, a.D - 1);
+    const double raw = a.q[min(qrow, a.nq - 1) * a.D + cc] - a.frame[cc];
+    const _Float16 v = screen_f16((qvalid && c < a.D) ? raw : 0.0, sigma);
+    sc.qh[s / 8][s % 8] = (_Float16)(-2.0f * (float)v);  // exact
     qn_part += (double)v * (double)v;
   }
   {
-    const float qn = (float)(qn_part + __shfl_xor(qn_part, 32));
-    const __bf16 qnh = (__bf16)qn;
-    const __bf16 qnl = (__bf16)(qn - (float)qnh);
-    const uint32_t qw = (uint32_t)__builtin_bit_cast(uint16_t, qnh) |
-                        ((uint32_t)__builtin_bit_cast(uint16_t, qnl) << 16);
-    sc.qnb = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(qw, 0x3F803F80u, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u));
+    // The query's norm as three bf16 pieces (as screen_norm_words), packed
+    // for the query side of the norm block.
+    const uint2 w = screen_norm_words((float)(qn_part + __shfl_xor(qn_part, 32)));
+    const uint32_t q12 = (w.x >> 16) | (w.y << 16);               // q1 | q2 << 16
+    const uint32_t q3one = (w.y >> 16) | 0x3F800000u;             // q3 | 1 << 16
+    sc.qnb = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(q12, q3one, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
   }
 #pragma unroll
   for (int e = 0; e < KP; ++e) {
@@ -391,7 +387,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   } while (0)
   if (nsteps > 0) {  // uniform over the workgroup
     uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
-    uint32_t xp = 0, yp = 0;
+    uint2 xp = {}, yp = {};
     stage_load<KS>(a, 0, t1, tb, ts, tid, x0, x1, xp);
     stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
     stage_load<KS>(a, min(1u, nsteps - 1) * TS, t1, tb, ts, tid, x0, x1, xp);
@@ -425,8 +421,9 @@ struct SelArgs {
   uint8_t* __restrict__ flag;
   uint32_t* __restrict__ nflag;  // queries flagged so far (cleared before the select)
   uint64_t nq;
+  const double* __restrict__ frame;  // [D + 1] the tree's centre and sigma
   uint32_t D, k, lists;
-  double pmax, cutoff, max_abs;
+  double sigma, pc, cutoff;  // sigma, P_c >= max over rows of |p - c|, the reduced cutoff
 };
 
 // One wave per query; lane l reads lists l, l + 64, ... (at most LISTS_PER_LANE):
@@ -469,35 +466,39 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
       }
     }
   }
-  double qn = 0.0, qmax = 0.0;
+  // |q - c| and the largest |fl64(q_j - c_j)|, in the frame of the rows.
+  double qc2 = 0.0, qmax = 0.0;
   for (uint32_t c = lane; c < a.D; c += 64) {
-    const double v = a.q[qi * a.D + c];
-    const double f = (double)(float)v;
-    qn += f * f;
-    qmax = fmax(qmax, fabs(v));
+    const double y = a.q[qi * a.D + c] - a.frame[c];
+    qc2 += y * y;
+    qmax = fmax(qmax, fabs(y));
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    qn += __shfl_xor(qn, o);
+    qc2 += __shfl_xor(qc2, o);
     qmax = fmax(qmax, __shfl_xor(qmax, o));
   }
-  // tau_A (every lane computes the same value)
-  const double u = 0x1p-24;
+  // tau_A (every lane computes the same value).  Each bound is rounded up by
+  // a relative 1e-12 (far above the FP64 error of computing it).
   const double D = (double)a.D;
-  const double m3 = (3.0 * D + 4.0) * u;
-  const double g = 3.1 * 0x1p-16 + 1.03 * m3 / (1.0 - m3);  // split residual + FP32 accumulation
-  const double N = (sqrt(qn) * (1.0 + 0x1p-50) + a.pmax);
-  const double E2 = ((2.0 * g + 8.0 * u) * N * N + D * 0x1p-110 * (1.0 + N) * (1.0 + N)) * (1.0 + 1e-12);
-  const double e1 = (u * (1.0 + 2.0 * u) * N + sqrt(D) * 0x1p-148) * (1.0 + 1e-12);
+  const double sg = a.sigma;
+  const double qc = sqrt(qc2) * (1.0 + 0x1p-44);  // >= |q - c| (as P_c in the build)
+  const double ea = 0x1p-11 * (1.0 + 0x1p-12);
+  const double eb = 0x1p-14 * (1.0 + 0x1p-60) * 2.0 * sqrt(D) * (1.0 + 0x1p-50);
+  const double e1 = (ea * sg * (qc + a.pc) + eb) * (1.0 + 1e-12);
+  const double Nh = (sg * (qc + a.pc) * (1.0 + ea) + eb) * (1.0 + 1e-12);
+  const double m = (D + 5.0) * 0x1p-23;
+  const double gam = m / (1.0 - m);
+  const double E2 = (((1.0 + 0x1p-7) * gam + 1.13 * 0x1p-24) * Nh * Nh + 0x1p-118) * (1.0 + 1e-12);
   const double sh = 1.0 + 2.0 * (D + 3.0) * 0x1p-53;
   double tau = a.cutoff;
   if (ak < INFINITY) {
-    const double r = sqrt(fmax(0.0, (double)ak + E2)) + e1;
+    const double r = (sqrt(fmax(0.0, (double)ak + E2)) + e1) / sg;
     tau = fmin(tau, sh * r * r * (1.0 + 1e-12));
   }
   float ta = INFINITY;
   if (tau < INFINITY) {
-    const double r = sqrt(sh * tau) + e1;
+    const double r = sg * sqrt(sh * tau) + e1;
     const double t = (r * r + E2) * (1.0 + 1e-12);
     ta = (float)t;
     if ((double)ta < t) ta = nextafterf(ta, INFINITY);
@@ -521,7 +522,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
     cnt_c[c] = n;
     mcount += n;
   }
-  const bool ok = qmax <= 1e18 && a.max_abs <= 1e18 && !__ballot(!complete);
+  const bool ok = qmax * sg <= 2048.0 && !__ballot(!complete);  // in range of the fp16 rounding (NaN: not)
   // Exclusive prefix sum of the per-lane counts.
   uint32_t incl = mcount;
 #pragma unroll
@@ -761,7 +762,7 @@ uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.allow_scratch && qp.metric == M_L2 && qp.eps_factor == 1.0 &&
-         qp.k <= (uint64_t)KMAX && t.rows_bf &&
+         qp.k <= (uint64_t)KMAX && t.rows_h &&
          qp.nq < (1ull << 31) && (t.d >= 32 || (qp.matrix_screen && t.d >= 8)) && t.d <= 64;
 }
 
@@ -800,8 +801,9 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     return 100;
   }
   SArgs sa{};
-  sa.rows_bf = t.rows_bf;
-  sa.pnorm_bf = t.pnorm_bf;
+  sa.rows_h = t.rows_h;
+  sa.pnorm_h = t.pnorm_h;
+  sa.frame = t.frame;
   sa.q = trav.q;
   sa.part_a = part_a;
   sa.part_i = part_i;
@@ -837,9 +839,10 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   se.D = (uint32_t)t.d;
   se.k = k;
   se.lists = (uint32_t)(2 * slices);
-  se.pmax = t.pmax;
+  se.frame = t.frame;
+  se.sigma = t.scr_sigma;
+  se.pc = t.scr_pc;
   se.cutoff = qp.cutoff;
-  se.max_abs = t.max_abs;
   launch_zero_u32(nflag, 1, s);
   {
     KernelTimer tm(s);

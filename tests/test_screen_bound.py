@@ -1,30 +1,38 @@
 """The matrix-core screen's error bound, checked against exact arithmetic.
 
-The screen (kdt_screen.hip) rounds the rows to FP32 (q~, p~) and their squared
-norms to FP32 (qn, pn), splits each value x into bf16 halves xh = bf16(x),
-xl = bf16(x - xh), and has the matrix cores sum in FP32 the 3D exact products
-of -2 q~ and p~ (xh.yh + xl.yh + xh.yl per coordinate) and the four exact terms
-qn_h + qn_l + pn_h + pn_l: that sum is A.  It relies on two consequences of
-its bound, with N = |q~| + |p~|, u = 2^-24, s = 1 + 2(D+3)2^-53,
-g = 3.1 2^-16 + 1.03 (3D+4)u / (1 - (3D+4)u):
+The screen (kdt_screen.hip) moves every row into a frame fixed per tree, centre
+c (the middle of the root box) and a power of two sigma with sigma |x_j - c_j|
+< 2^8 over the tree, and rounds it to fp16: x^_j = fp16(fl32(sigma fl64(x_j -
+c_j))), nearest even, fp16 subnormals flushed to zero.  The matrix cores sum in
+FP32 the D exact products (-2 q^_j) p^_j and the norms qn = fl32(|q^|^2), pn =
+fl32(|p^|^2), each as three bf16 pieces (p1 = bf16(pn), p2 = bf16(pn - p1),
+p3 = bf16(pn - p1 - p2)): that sum is A.  With a = 2^-11 (1 + 2^-12),
+b = 2^-14, P_c >= |p - c| over the tree, s = 1 + 2(D+3)2^-53:
 
-  (upper)     d <= U(A) = s (sqrt(max(A, 0) + E2) + e1)^2
-  (inclusion) A <= T(d) = (sqrt(s d) + e1)^2 + E2
-  E2 = (2g + 8u) N^2 + D 2^-110 (1 + N)^2,   e1 = u (1 + 2u) N + sqrt(D) 2^-148
+  per coordinate   |sigma (x_j - c_j) - x^_j| <= a sigma |x_j - c_j| + b
+  e1 = a sigma (|q - c| + P_c) + 2 b sqrt(D)
+  Nh = sigma (|q - c| + P_c)(1 + a) + 2 b sqrt(D)
+  E2 = ((1 + 2^-7) gamma + 1.13 2^-24) Nh^2 + 2^-118,
+       gamma = (D + 5) 2^-23 / (1 - (D + 5) 2^-23)
+  (core)       |A - |q^ - p^|^2| <= E2
+  (upper)      d <= U(A) = s (sqrt(max(A, 0) + E2) + e1)^2 / sigma^2
+  (inclusion)  A <= T(d) = (sigma sqrt(s d) + e1)^2 + E2
 
-where d is the reference's FP64 reduced distance (the oracle's point_dist, the
-Streamed / Folded summation of src/kernel.rs).  (upper) makes tau an upper bound
-of the k-th distance; (inclusion) puts every top-k point among the candidates.
-Here A is formed as the kernel forms it (bf16 rounding emulated bit for bit,
-round to nearest even), with the 3D + 4 terms accumulated in FP32 in several
-orders (the matrix cores' internal order is not specified), on inputs chosen to
-stress the bound: near-equal rows with large norms (cancellation in
-|q|^2 + |p|^2 - 2 q.p), rows whose split residuals all add up, mixed
-magnitudes, values near FP32 underflow and near the 1e18 limit of the screen.
-The exact |q~ - p~|^2 (rationals) checks the core inequality
-|A - |q~ - p~|^2| <= E2 itself, and bounds without the cancellation term or
-without the split term must each fail on these inputs, so the test can tell a
-wrong bound.
+where d is the reference's FP64 reduced distance (the oracle's point_dist).
+(upper) makes tau an upper bound of the k-th distance; (inclusion) puts every
+top-k point among the candidates.  Here the rounding is emulated bit for bit
+(numpy's float64 -> float32 -> float16 conversions round to nearest even) and
+A is accumulated in FP32 in several orders, and once rounding every addition
+toward zero (the bound allows 2^-23 per addition; the matrix cores' internal
+order and rounding are not specified).  The inputs stress each term: rows far
+from the centre but close to each other (the fp16 rounding of both rows is the
+whole distance error), near-equal rows with large norms (cancellation in
+|q|^2 + |p|^2 - 2 q.p), coordinates below the fp16 normal range (flushed),
+mixed magnitudes, offset data (the centre removes the offset), huge values
+(no range limit on the data: sigma scales them) and queries near the 2^11
+limit.  Bounds without the fp16 term or without the accumulation term must
+each fail on these inputs, so the test can tell a wrong bound (the flush term
+is dominated by the fp16 term in every frame; a test shows why).
 """
 
 from __future__ import annotations
@@ -37,11 +45,8 @@ import pytest
 
 from oracle import point_dist
 
-U = 2.0 ** -24
-
-
-def f32(x):
-    return np.float32(x)
+A_REL = 2.0 ** -11 * (1 + 2.0 ** -12)
+B_ABS = 2.0 ** -14 * (1 + 2.0 ** -60)
 
 
 def bf16(x) -> np.float32:
@@ -51,118 +56,211 @@ def bf16(x) -> np.float32:
     return np.array([r & 0xFFFFFFFF], dtype=np.uint32).view(np.float32)[0]
 
 
-def split_terms(a, b):
-    """The 3 D exact FP32 products of the bf16 split, per coordinate."""
-    out = []
-    for x, y in zip(a, b):
-        xh, yh = bf16(x), bf16(y)
-        xl, yl = bf16(f32(x - xh)), bf16(f32(y - yh))
-        out += [f32(float(xh) * float(yh)), f32(float(xl) * float(yh)), f32(float(xh) * float(yl))]
-    return out
+def frame(tree_rows: np.ndarray):
+    """Centre, sigma and P_c as the build makes them (k_screen_frame, k_rows_h)."""
+    lo, hi = tree_rows.min(axis=0), tree_rows.max(axis=0)
+    c = 0.5 * lo + 0.5 * hi
+    m = float(np.max(np.maximum(np.abs(hi - c), np.abs(lo - c))))
+    s = min(8 - (math.frexp(m)[1]), 1000) if m > 0 else 0  # frexp exponent = ilogb + 1
+    sigma = math.ldexp(1.0, s)
+    pc = max(math.sqrt(float(np.sum((r - c) ** 2))) for r in tree_rows) * (1 + 2.0 ** -44)
+    return c, sigma, pc
+
+
+def f16_row(x: np.ndarray, c: np.ndarray, sigma: float) -> np.ndarray:
+    """screen_f16 of every coordinate: fp16(fl32(sigma fl64(x - c))), clamped
+    to +-2^11, fp16 subnormals flushed; as float64 values."""
+    v = np.clip(((x - c) * sigma).astype(np.float32), -2048.0, 2048.0)
+    h = v.astype(np.float16).astype(np.float64)
+    h[np.abs(h) < 2.0 ** -14] = 0.0
+    return h
+
+
+def pieces(h: np.ndarray):
+    pn = np.float32(float(np.sum(h * h)))
+    p1 = bf16(pn)
+    r1 = np.float32(pn - p1)
+    p2 = bf16(r1)
+    p3 = bf16(np.float32(r1 - p2))
+    return [p1, p2, p3]
+
+
+def rtz32(x: Fraction) -> np.float32:
+    """x rounded toward zero to float32."""
+    f = np.float32(float(x))
+    zero = np.float32(0.0)
+    while abs(Fraction(float(f))) > abs(x):
+        f = np.nextafter(f, zero)
+    away = np.float32(np.inf) if x > 0 else np.float32(-np.inf)
+    while True:
+        g = np.nextafter(f, away)
+        if abs(Fraction(float(g))) <= abs(x):
+            f = g
+        else:
+            return f
 
 
 def fsum32(v, order):
     """FP32 sum of float32 terms in a given order."""
     if order == "seq":
-        acc = f32(0.0)
+        acc = np.float32(0.0)
         for t in v:
-            acc = f32(acc + t)
+            acc = np.float32(acc + t)
         return acc
-    if order == "fma":  # one rounding per step of a wider sum
-        acc = 0.0
+    if order == "rtz":
+        acc = np.float32(0.0)
         for t in v:
-            acc = float(f32(acc + float(t)))
-        return f32(acc)
+            acc = rtz32(Fraction(float(acc)) + Fraction(float(t)))
+        return acc
     if order == "pairs":
-        acc = f32(0.0)
+        acc = np.float32(0.0)
         for i in range(0, len(v), 2):
-            t = f32(v[i] + v[i + 1]) if i + 1 < len(v) else v[i]
-            acc = f32(acc + t)
+            t = np.float32(v[i] + v[i + 1]) if i + 1 < len(v) else v[i]
+            acc = np.float32(acc + t)
         return acc
     if order == "tree":
         while len(v) > 1:
-            v = [f32(v[i] + v[i + 1]) if i + 1 < len(v) else v[i] for i in range(0, len(v), 2)]
+            v = [np.float32(v[i] + v[i + 1]) if i + 1 < len(v) else v[i] for i in range(0, len(v), 2)]
         return v[0]
     raise ValueError(order)
 
 
-def screen_A(q32, p32, order):
-    qn = f32(float(np.sum(q32.astype(np.float64) ** 2)))
-    pn = f32(float(np.sum(p32.astype(np.float64) ** 2)))
-    norms = []
-    for x in (qn, pn):  # the norm block: x_h * 1 + x_l * 1
-        xh = bf16(x)
-        norms += [xh, bf16(f32(x - xh))]
-    prods = split_terms((np.float32(-2.0) * q32).astype(np.float32), p32)
+def screen_A(hq, hp, order):
+    prods = [np.float32((-2.0 * a) * b) for a, b in zip(hq, hp)]  # exact: 22-bit products
+    assert all(float(t) == (-2.0 * a) * b for t, a, b in zip(prods, hq, hp))
+    norms = pieces(hq) + pieces(hp)
     if order == "norms-first":
         return float(fsum32(norms + prods, "seq"))
     return float(fsum32(prods + norms, order))  # the kernel issues the norm block last
 
 
-def bounds(D, nq, npn, cancellation=True, split=True):
-    m3 = (3.0 * D + 4.0) * U
-    g = (3.1 * 2.0 ** -16 if split else 0.0) + 1.03 * m3 / (1.0 - m3)
-    N = nq * (1.0 + 2.0 ** -50) + npn
-    E2 = ((2.0 * g + 8.0 * U) * N * N if cancellation else 8.0 * U * max(nq, npn) ** 2)
-    E2 += D * 2.0 ** -110 * (1.0 + N) ** 2
-    e1 = U * (1.0 + 2.0 * U) * N + math.sqrt(D) * 2.0 ** -148
+def bounds(D, qc, pc, sigma, fp16_term=True, flush_term=True, accumulation=True):
+    a = A_REL if fp16_term else 2.0 ** -24
+    b = B_ABS if flush_term else 0.0
+    eb = b * 2.0 * math.sqrt(D) * (1 + 2.0 ** -50)
+    e1 = (a * sigma * (qc + pc) + eb) * (1 + 1e-12)
+    nh = (sigma * (qc + pc) * (1 + a) + eb) * (1 + 1e-12)
+    m = (D + 5.0) * 2.0 ** -23
+    gam = m / (1 - m) if accumulation else 0.0
+    E2 = (((1 + 2.0 ** -7) * gam + 1.13 * 2.0 ** -24) * nh * nh + 2.0 ** -118) * (1 + 1e-12)
     s = 1.0 + 2.0 * (D + 3.0) * 2.0 ** -53
-    return E2 * (1 + 1e-12), e1 * (1 + 1e-12), s
+    return E2, e1, s
 
 
 def cases():
+    """(D, tree rows, q, p): p is a row of the tree; q any query."""
     rng = np.random.default_rng(2024)
     out = []
     for D in (16, 33, 64):
-        for _ in range(40):  # ordinary Gaussian rows
-            out.append((D, rng.standard_normal(D), rng.standard_normal(D)))
-        for scale in (1.0, 1e3, 1e9, 1e17):  # near-equal rows, large norms: cancellation
+        for _ in range(30):  # ordinary Gaussian rows
+            t = rng.standard_normal((4, D))
+            out.append((D, t, rng.standard_normal(D), t[0]))
+        for scale in (1.0, 1e3, 1e17, 1e150):
+            # rows far from the centre, close to each other: the fp16 rounding of
+            # both rows is the distance error (and not cancelled by the centre)
             base = rng.standard_normal(D) * scale
-            for eps in (1e-7, 1e-5, 1e-3):
-                out.append((D, base, base + eps * scale * rng.standard_normal(D)))
-        # mixed magnitudes, values near FP32 underflow, the 1e18 screen limit
-        out.append((D, rng.standard_normal(D) * np.logspace(-30, 10, D), rng.standard_normal(D)))
-        out.append((D, rng.standard_normal(D) * 1e-38, rng.standard_normal(D) * 1e-39))
-        out.append((D, np.full(D, 9.9e17), np.full(D, 9.9e17) * (1 + 1e-7)))
-        # the FP32 value whose split contraction x.x is worst (relative error
-        # 1.99 2^-16, found by exhausting [1, 2)), in every coordinate, so that
-        # the split errors add up; as duplicates (A ~ 0) and opposite rows
-        x = float(np.array([0x3F808040], dtype=np.uint32).view(np.float32)[0])
-        for scale in (1.0, 2.0 ** -40, 2.0 ** 50):
-            out.append((D, np.full(D, x * scale), np.full(D, x * scale)))
-            out.append((D, np.full(D, x * scale), np.full(D, -x * scale)))
+            t = np.stack([base, -base, base * 0.5, -base * 0.5])
+            for eps in (1e-7, 1e-4, 1e-2):
+                out.append((D, t, base + eps * scale * rng.standard_normal(D), base))
+            # near-equal rows with large norms, data offset from the origin
+            t = base + eps * scale * rng.standard_normal((4, D))
+            out.append((D, t, t[1] + 1e-3 * eps * scale, t[0]))
+        # coordinates below the fp16 normal range after scaling (flushed)
+        t = rng.standard_normal((4, D)) * np.logspace(-12, 0, D)
+        out.append((D, t, rng.standard_normal(D) * np.logspace(-12, 0, D), t[2]))
+        tiny = np.zeros((4, D))
+        tiny[0, 0] = 1.0  # sets sigma; every other coordinate flushes
+        tiny[1:, 1:] = rng.standard_normal((3, D - 1)) * 2.0 ** -24
+        out.append((D, tiny, np.full(D, 2.0 ** -25), tiny[1]))
+        # mixed magnitudes, huge values, queries near the 2^11 limit
+        t = rng.standard_normal((4, D)) * np.logspace(-30, 10, D)
+        out.append((D, t, rng.standard_normal(D) * np.logspace(-30, 10, D), t[3]))
+        t = rng.standard_normal((4, D)) * 1e150  # (squared distances near 1e302)
+        out.append((D, t, rng.standard_normal(D) * 1e150, t[0]))
+        t = rng.uniform(-1.0, 1.0, (4, D))
+        out.append((D, t, np.full(D, 7.9), t[1]))  # sigma = 2^8: 7.9 sigma = 2022 < 2^11
+        # duplicates and exact opposites
+        t = rng.standard_normal((4, D))
+        out.append((D, t, t[0].copy(), t[0]))
+        out.append((D, t, -t[0], t[0]))
     return out
 
 
-@pytest.mark.parametrize("order", ["seq", "fma", "pairs", "tree", "norms-first"])
+def check_case(D, tree_rows, q, p, order, **weak):
+    c, sigma, pc = frame(tree_rows)
+    ymax = float(np.max(np.abs(q - c))) * sigma
+    if not ymax <= 2048.0:
+        return None  # the select flags this query: no bound is claimed
+    hq, hp = f16_row(q, c, sigma), f16_row(p, c, sigma)
+    qc = math.sqrt(float(np.sum((q - c) ** 2))) * (1 + 2.0 ** -44)
+    E2, e1, s = bounds(D, qc, pc, sigma, **weak)
+    return hq, hp, c, sigma, qc, pc, E2, e1, s
+
+
+@pytest.mark.parametrize("order", ["seq", "rtz", "pairs", "tree", "norms-first"])
 def test_screen_bound_holds(order):
-    for D, q, p in cases():
-        q32, p32 = q.astype(np.float32), p.astype(np.float32)
-        A = screen_A(q32, p32, order)
-        nq = float(np.linalg.norm(q32.astype(np.float64)))
-        npn = float(np.linalg.norm(p32.astype(np.float64)))
-        E2, e1, s = bounds(D, nq, npn)
-        exact = sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(q32, p32))
+    checked = 0
+    for D, tree_rows, q, p in cases():
+        r = check_case(D, tree_rows, q, p, order)
+        if r is None:
+            continue
+        hq, hp, c, sigma, qc, pc, E2, e1, s = r
+        # per coordinate, exactly: |sigma (x - c) - x^| <= a sigma |x - c| + b
+        for x, h in ((q, hq), (p, hp)):
+            for xj, cj, hj in zip(x, c, h):
+                y = (Fraction(float(xj)) - Fraction(float(cj))) * Fraction(sigma)
+                assert abs(y - Fraction(float(hj))) <= Fraction(A_REL) * abs(y) + Fraction(B_ABS)
+        A = screen_A(hq, hp, order)
+        exact = sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(hq, hp))
         assert abs(Fraction(A) - exact) <= Fraction(E2), (D, order)
         d = point_dist(q, p)  # the reference's FP64 reduced distance
-        upper = s * (math.sqrt(max(A, 0.0) + E2) + e1) ** 2 * (1 + 1e-12)
+        upper = s * ((math.sqrt(max(A, 0.0) + E2) + e1) / sigma) ** 2 * (1 + 1e-12)
         assert d <= upper, (D, order, d, upper)
-        incl = ((math.sqrt(s * d) + e1) ** 2 + E2) * (1 + 1e-12)
+        incl = ((sigma * math.sqrt(s * d) + e1) ** 2 + E2) * (1 + 1e-12)
         assert A <= incl, (D, order, A, incl)
+        checked += 1
+    assert checked >= 150
 
 
-@pytest.mark.parametrize("weakened", ["no-cancellation-term", "no-split-term"])
+def test_far_queries_are_not_screened():
+    # rows spanning [-1, 1] in every coordinate: M = 1, sigma = 2^7, so a query
+    # coordinate at 16.1 maps beyond 2^11 and one at 15.9 does not
+    rng = np.random.default_rng(7)
+    t = rng.uniform(-1.0, 1.0, (4, 16))
+    t[0], t[1] = -1.0, 1.0
+    assert frame(t)[1] == 2.0 ** 7
+    assert check_case(16, t, np.full(16, 16.1), t[2], "seq") is None
+    assert check_case(16, t, np.full(16, 15.9), t[2], "seq") is not None
+
+
+def test_flush_term_is_dominated():
+    """The flushed-coordinate term 2 b sqrt(D) of e1 never exceeds the fp16 term
+    (for data spanning more than 2^-990, where sigma is not capped at 2^1000):
+    sigma makes the tree's largest |x_j - c_j| at least 2^7 / sigma, so
+    a sigma P_c >= 2^-11 2^7 = 2^-4 > 2 b sqrt(64) = 2^-10.  (It stays in the
+    bound, which is derived without this property; no input can make a bound
+    without it fail.)"""
+    for D, tree_rows, q, p in cases():
+        c, sigma, pc = frame(tree_rows)
+        assert sigma * pc >= 2.0 ** 7 or pc == 0.0
+        assert A_REL * sigma * pc >= 2 * B_ABS * math.sqrt(D) or pc == 0.0
+
+
+@pytest.mark.parametrize("weakened", ["fp16_term", "accumulation"])
 def test_weakened_bound_fails(weakened):
-    """Dropping the 2 g N^2 term (the error of q.p under cancellation), or only
-    its bf16 split part, must break the core inequality on these inputs."""
+    """A bound that treats the rows as exact FP32 values (no fp16 term), or that
+    leaves out the FP32 accumulation, must break (core), (upper) or
+    (inclusion) on these inputs."""
     broken = 0
-    for D, q, p in cases():
-        q32, p32 = q.astype(np.float32), p.astype(np.float32)
-        A = screen_A(q32, p32, "seq")
-        nq = float(np.linalg.norm(q32.astype(np.float64)))
-        npn = float(np.linalg.norm(p32.astype(np.float64)))
-        E2, _, _ = bounds(D, nq, npn, cancellation=weakened != "no-cancellation-term",
-                          split=weakened != "no-split-term")
-        exact = sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(q32, p32))
-        broken += abs(Fraction(A) - exact) > Fraction(E2)
+    for D, tree_rows, q, p in cases():
+        r = check_case(D, tree_rows, q, p, "seq", **{weakened: False})
+        if r is None:
+            continue
+        hq, hp, c, sigma, qc, pc, E2, e1, s = r
+        A = screen_A(hq, hp, "seq")
+        exact = sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(hq, hp))
+        d = point_dist(q, p)
+        incl = ((sigma * math.sqrt(s * d) + e1) ** 2 + E2) * (1 + 1e-12)
+        upper = s * ((math.sqrt(max(A, 0.0) + E2) + e1) / sigma) ** 2 * (1 + 1e-12)
+        broken += abs(Fraction(A) - exact) > Fraction(E2) or A > incl or d > upper
     assert broken > 0

@@ -65,6 +65,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "kdt_query_kernels.h"
 
@@ -90,8 +91,8 @@ constexpr uint32_t MAX_SLICES = 32 * LISTS_PER_LANE;
 struct SArgs {
   const uint16_t* __restrict__ rows_h;  // [n_pad][2 KS] fp16, tree order, zero-padded
   const uint2* __restrict__ pnorm_h;    // [n_pad] their norms, norm-block words (+inf on padding rows)
-  const double* __restrict__ frame;     // [D + 1] centre, sigma
-  const double* __restrict__ q;      // [nq][D]
+  const uint4* __restrict__ qrows;      // [nq][KS / 4] the queries as -2 q^, fp16 (k_screen_queries)
+  const uint2* __restrict__ qnorm;      // [nq] their norm-block words, query side
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
   uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
   uint64_t nq;
@@ -100,20 +101,33 @@ struct SArgs {
   uint32_t interleave;       // k > KP: slice y takes tiles y, y + slices, ...
 };
 
-// Waves (32-query tiles) per workgroup sharing each point tile.  Rows of 64
-// columns: 8 measured fastest (4: each point tile read twice as often from L2,
-// 33.0 vs 22.8 ms; 16: fewer workgroups than CUs can hold, a partial second
-// round).  Rows of 16 and 32 columns: 4 (the per-tile barrier waits on fewer
-// waves; 16-d uniform rows 1.8 vs 2.3 ms, 16-d normal rows 3% faster; 24-d
-// uniform rows at k = 32 5.1 vs 6.2 ms, 32-d normal rows at k = 10 3% slower).
+// Query tiles (32 queries each) per wave.  Two would share each point
+// operand read from LDS between two independent accumulator chains, but their
+// registers (186 VGPRs) leave 2 waves per SIMD instead of 4: 64-d rows, 2^20
+// points, 65,536 queries 19.1 vs 13.5 ms with one tile.
+#ifndef KDT_SCREEN_QT32
+#define KDT_SCREEN_QT32 1
+#endif
 template <int KS>
-constexpr int screen_waves() { return KS <= 16 ? 4 : 8; }
+constexpr int screen_qt() { return KS <= 16 ? 1 : KDT_SCREEN_QT32; }
+
+// Waves per workgroup sharing each point tile.  Rows of 16 and 32 columns: 4
+// (the per-tile barrier waits on fewer waves; 16-d uniform rows 1.8 vs 2.3 ms,
+// 16-d normal rows 3% faster; 24-d uniform rows at k = 32 5.1 vs 6.2 ms, 32-d
+// normal rows at k = 10 3% slower).  Rows of 64 columns: 8 query tiles per
+// workgroup (with 4, each point tile is read twice as often from L2).
+#ifndef KDT_SCREEN_SW32
+#define KDT_SCREEN_SW32 (8 / KDT_SCREEN_QT32)
+#endif
+template <int KS>
+constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
 
 // Point tiles per pipeline step: the TS tiles of a step are staged together
-// and contracted between two barriers.  Narrow rows make short tiles (4
-// matrix instructions for rows of up to 16 columns against 13 for 64), so a
-// barrier and a staging round per tile leave the matrix cores idle; a step of
-// several tiles amortises them.
+// and contracted between two barriers, which amortises the barrier and the
+// staging round over TS tiles.  Rows of 64 columns: 2 (5 matrix instructions
+// per tile; 2^20 points, k = 10: 65,536 queries 13.07 vs 13.48 ms, 8,192
+// queries 2.37 vs 2.57 ms, 1,024 queries 0.58 vs 0.66 ms).  Rows of 16 and 32
+// columns: 1.
 #ifndef KDT_SCREEN_TS8
 #define KDT_SCREEN_TS8 1
 #endif
@@ -121,7 +135,7 @@ constexpr int screen_waves() { return KS <= 16 ? 4 : 8; }
 #define KDT_SCREEN_TS16 1
 #endif
 #ifndef KDT_SCREEN_TS32
-#define KDT_SCREEN_TS32 1
+#define KDT_SCREEN_TS32 2
 #endif
 template <int KS>
 constexpr int screen_tiles_per_step() { return KS <= 8 ? KDT_SCREEN_TS8 : KS <= 16 ? KDT_SCREEN_TS16 : KDT_SCREEN_TS32; }
@@ -208,90 +222,124 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, co
 #ifndef KDT_SCREEN_IB8
 #define KDT_SCREEN_IB8 16
 #endif
+#ifndef KDT_SCREEN_IB32
+#define KDT_SCREEN_IB32 12
+#endif
 template <int KS>
-constexpr uint32_t insert_slots() { return KS <= 8 ? KDT_SCREEN_IB8 : 12; }
+constexpr uint32_t insert_slots() { return KS <= 8 ? KDT_SCREEN_IB8 : KS <= 16 ? 12 : KDT_SCREEN_IB32; }
 constexpr int CHK = 8;
-static_assert(KDT_SCREEN_IB8 > CHK, "a buffer holds more than one check interval");
+static_assert(KDT_SCREEN_IB8 > CHK && KDT_SCREEN_IB32 > CHK, "a buffer holds more than one check interval");
 
+// The queries in the screen's operand form, once per call (every point
+// slice reads them): row i as -2 q^ (q^_j = screen_f16(fl64(q_j - c_j),
+// sigma), exact in fp16) over 2 KS columns (zero past D), and the norm block's
+// query-side words of qn = fl32(|q^|^2) (summed in FP64): (q1 | q2 << 16,
+// q3 | 1 << 16), from the three bf16 pieces of screen_norm_words.
+template <int KS>
+__global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint32_t D,
+                                 const double* __restrict__ frame, uint16_t* __restrict__ qrows,
+                                 uint2* __restrict__ qnorm) {
+  const double sigma = frame[D];
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) {
+    double qn = 0.0;
+    for (uint32_t c = 0; c < 2 * KS; ++c) {
+      const _Float16 v = c < D ? screen_f16(q[i * D + c] - frame[c], sigma) : (_Float16)0.0f;
+      qrows[i * 2 * KS + c] = __builtin_bit_cast(uint16_t, (_Float16)(-2.0f * (float)v));
+      qn += (double)v * (double)v;
+    }
+    const uint2 w = screen_norm_words((float)qn);
+    qnorm[i] = make_uint2((w.x >> 16) | (w.y << 16), (w.y >> 16) | 0x3F800000u);
+  }
+}
+
+// One wave's state for its QT query tiles: per tile the query operands, the
+// lane's list and its insert buffer (tile u, slot e at buf[(u IB + e) 64 SW]).
 template <int KS>
 struct Screen {
   static constexpr int NS = KS / 8;  // f16x8 operands per lane half
   static constexpr int RU = KS / 4, RS = RU + 1;
-  static constexpr uint32_t IB = insert_slots<KS>();
-  f16x8 qh[NS];     // -2 q^
-  bf16x8 qnb;       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
-  float la[KP];     // the KP smallest A seen by this lane, ascending
-  uint32_t li[KP];  // their tree positions
-  float thr;        // la[KP - 1] at the last merge
-  uint32_t cnt;     // entries in this lane's buffer
-  uint2* buf;       // this lane's buffer: slot e at buf[e * 64 * SW], (A bits, position)
+  static constexpr int QT = screen_qt<KS>();
   static constexpr int SW = screen_waves<KS>();
+  static constexpr uint32_t IB = insert_slots<KS>();
+  f16x8 qh[QT][NS];     // -2 q^
+  bf16x8 qnb[QT];       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
+  float la[QT][KP];     // the KP smallest A seen by this lane, ascending
+  uint32_t li[QT][KP];  // their tree positions
+  float thr[QT];        // la[u][KP - 1] at the last merge
+  uint32_t cnt[QT];     // entries in this lane's buffer
+  uint2* buf;
 
-  __device__ __forceinline__ void insert(float A, uint32_t row) {  // dropping the largest
+  __device__ __forceinline__ void insert(int u, float A, uint32_t row) {  // dropping the largest
 #pragma unroll
     for (int e = KP - 1; e > 0; --e) {
-      const bool up = A < la[e - 1];
-      const bool here = !up && A < la[e];
-      la[e] = up ? la[e - 1] : (here ? A : la[e]);
-      li[e] = up ? li[e - 1] : (here ? row : li[e]);
+      const bool up = A < la[u][e - 1];
+      const bool here = !up && A < la[u][e];
+      la[u][e] = up ? la[u][e - 1] : (here ? A : la[u][e]);
+      li[u][e] = up ? li[u][e - 1] : (here ? row : li[u][e]);
     }
-    if (A < la[0]) {
-      la[0] = A;
-      li[0] = row;
+    if (A < la[u][0]) {
+      la[u][0] = A;
+      li[u][0] = row;
     }
   }
 
-  // Merge every lane's buffer into its list; the whole wave calls it.
-  __device__ __forceinline__ void merge() {
+  // Merge every lane's buffer of tile u into its list; the whole wave calls it.
+  __device__ __forceinline__ void merge(int u) {
 #pragma unroll 1
     for (uint32_t e = 0; e < IB; ++e) {
-      if (!__builtin_amdgcn_ballot_w64(e < cnt)) break;
-      if (e < cnt) {
-        const uint2 v = buf[e * 64 * SW];
+      if (!__builtin_amdgcn_ballot_w64(e < cnt[u])) break;
+      if (e < cnt[u]) {
+        const uint2 v = buf[(u * IB + e) * 64 * SW];
         const float A = __uint_as_float(v.x);
-        if (A < la[KP - 1]) insert(A, v.y);
+        if (A < la[u][KP - 1]) insert(u, A, v.y);
       }
     }
-    thr = la[KP - 1];
-    cnt = 0;
+    thr[u] = la[u][KP - 1];
+    cnt[u] = 0;
   }
 
+  // The QT x 32 queries of the wave against point tile t (32 rows in LDS).
   __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
     const uint4* rb = rows + j * RS + h * NS;
     // Norm block, point side, of row j in half 0 (built with the rows).
     // Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
     const uint2 pw = pns[j];
     const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw.x, pw.y, 0u) : make_uint4(0u, 0u, 0u, 0u);
-    // The point operands are read one group ahead of their product.
-    floatx16 acc = {};
-    uint4 cur = rb[0];
+    floatx16 acc[QT];
+#pragma unroll
+    for (int u = 0; u < QT; ++u) acc[u] = floatx16{};
+    // Each point operand, read once, feeds the QT chains.
 #pragma unroll
     for (int v = 0; v < NS; ++v) {
-      uint4 nxt = cur;
-      if (v + 1 < NS) nxt = rb[v + 1];
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, cur), qh[v], acc, 0, 0, 0);
-      cur = nxt;
+      const f16x8 pv = __builtin_bit_cast(f16x8, rb[v]);
+#pragma unroll
+      for (int u = 0; u < QT; ++u) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pv, qh[u][v], acc[u], 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb, acc, 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < QT; ++u)
+      acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb[u], acc[u], 0, 0, 0);
     // Element r is row rowbase | (8 (r / 4) + r % 4): disjoint bits, one
     // immediate OR, nothing per element to keep in registers across tiles.
     const uint32_t rowbase = t * 32 + 4 * h;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float A = acc[r];
-      // A wave-uniform branch round the append: a plain divergent `if` here
-      // runs the append under an empty exec mask on every element instead of
-      // skipping it (the barrier keeps the two tests from being merged).
-      if (__builtin_amdgcn_ballot_w64(A < thr)) {
-        __builtin_amdgcn_sched_barrier(0);
-        if (A < thr) {
-          const uint32_t row = rowbase | (8 * (r / 4) + (r % 4));
-          buf[cnt * 64 * SW] = make_uint2(__float_as_uint(A), row);
-          ++cnt;
+    for (int u = 0; u < QT; ++u) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float A = acc[u][r];
+        // A wave-uniform branch round the append: a plain divergent `if` here
+        // runs the append under an empty exec mask on every element instead of
+        // skipping it (the barrier keeps the two tests from being merged).
+        if (__builtin_amdgcn_ballot_w64(A < thr[u])) {
+          __builtin_amdgcn_sched_barrier(0);
+          if (A < thr[u]) {
+            const uint32_t row = rowbase | (8 * (r / 4) + (r % 4));
+            buf[(u * IB + cnt[u]) * 64 * SW] = make_uint2(__float_as_uint(A), row);
+            ++cnt[u];
+          }
         }
+        // cnt <= IB - CHK before each group of CHK elements, so <= IB after it.
+        if (r % CHK == CHK - 1 && __builtin_amdgcn_ballot_w64(cnt[u] > IB - CHK)) merge(u);
       }
-      // cnt <= IB - CHK before each group of CHK elements, so <= IB after it.
-      if (r % CHK == CHK - 1 && __builtin_amdgcn_ballot_w64(cnt > IB - CHK)) merge();
     }
   }
 };
@@ -303,42 +351,31 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int TS = Stage<KS>::TS;
   __shared__ uint4 s_rows[2][TS * 32 * RS];
   __shared__ uint2 s_pn[2][TS * 32];
-  __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
+  constexpr int QT = screen_qt<KS>();
+  __shared__ uint2 s_buf[QT * insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
-  const uint64_t qtile = (uint64_t)blockIdx.x * SW + (tid >> 6);
-  const bool active = qtile * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
-  const uint64_t qrow = qtile * 32 + j;
-  const bool qvalid = qrow < a.nq;
+  // Wave w of the workgroup takes query tiles (blockIdx.x SW + w) QT + u.
+  const uint64_t qtile0 = ((uint64_t)blockIdx.x * SW + (tid >> 6)) * QT;
+  const bool active = qtile0 * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
   Screen<KS> sc;
-  const double sigma = a.frame[a.D];
-  double qn_part = 0.0;
+  constexpr int NS = KS / 8, RQ = KS / 4;
 #pragma unroll
-  for (int s = 0; s < KS; ++s) {
-    // Loads at clamped (in-bounds) indices, all issued before any is used;
-    // coordinates past D and rows past nq are zero.
-    const uint32_t c = h * KS + s;
-    const uint32_t cc = min(c, a.D - 1);
-    const double raw = a.q[min(qrow, a.nq - 1) * a.D + cc] - a.frame[cc];
-    const _Float16 v = screen_f16((qvalid && c < a.D) ? raw : 0.0, sigma);
-    sc.qh[s / 8][s % 8] = (_Float16)(-2.0f * (float)v);  // exact
-    qn_part += (double)v * (double)v;
-  }
-  {
-    // The query's norm as three bf16 pieces (as screen_norm_words), packed
-    // for the query side of the norm block.
-    const uint2 w = screen_norm_words((float)(qn_part + __shfl_xor(qn_part, 32)));
-    const uint32_t q12 = (w.x >> 16) | (w.y << 16);               // q1 | q2 << 16
-    const uint32_t q3one = (w.y >> 16) | 0x3F800000u;             // q3 | 1 << 16
-    sc.qnb = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(q12, q3one, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
-  }
+  for (int u = 0; u < QT; ++u) {
+    // Rows past nq read the last query (their lists are not written).
+    const uint64_t qrow = min((qtile0 + u) * 32 + j, a.nq - 1);
 #pragma unroll
-  for (int e = 0; e < KP; ++e) {
-    sc.la[e] = INFINITY;
-    sc.li[e] = 0xFFFFFFFFu;
+    for (int v = 0; v < NS; ++v) sc.qh[u][v] = __builtin_bit_cast(f16x8, a.qrows[qrow * RQ + h * NS + v]);
+    const uint2 w = a.qnorm[qrow];
+    sc.qnb[u] = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
+#pragma unroll
+    for (int e = 0; e < KP; ++e) {
+      sc.la[u][e] = INFINITY;
+      sc.li[u][e] = 0xFFFFFFFFu;
+    }
+    sc.thr[u] = INFINITY;
+    sc.cnt[u] = 0;
   }
-  sc.thr = INFINITY;
-  sc.cnt = 0;
   sc.buf = s_buf + tid;
 
   // Local tile i of slice y is global point tile tb + i ts.  The rows are in
@@ -401,13 +438,17 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   }
 #undef KDT_SCREEN_STEP
 #undef KDT_TILE
-  if (active) sc.merge();  // wave-uniform
-  if (qvalid) {
-    const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
 #pragma unroll
-    for (int e = 0; e < KP; ++e) {
-      a.part_a[base + e] = sc.la[e];
-      a.part_i[base + e] = sc.li[e];
+  for (int u = 0; u < QT; ++u) {
+    if (active) sc.merge(u);  // wave-uniform
+    const uint64_t qrow = (qtile0 + u) * 32 + j;
+    if (qrow < a.nq) {
+      const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
+#pragma unroll
+      for (int e = 0; e < KP; ++e) {
+        a.part_a[base + e] = sc.la[u][e];
+        a.part_i[base + e] = sc.li[u][e];
+      }
     }
   }
 }
@@ -743,19 +784,68 @@ __global__ __launch_bounds__(64) void k_screen_recheck(CheckArgs a) {
   }
 }
 
-// Point slices per query tile: enough waves to fill the device, at least 8
-// tiles per slice.  For k > KP, at least k / 4 interleaved slices: a query's
-// top k then spread over at least k / 2 lists of KP, about two per list, so a
-// list rarely fills below tau_A (the completeness check flags the query when
-// one does).  Returns the slice count; *per = tiles per slice.
+// Workgroups of k_screen<KS> the device holds at once (compute units x
+// workgroups per unit), looked up once per device.
+template <int KS>
+uint64_t screen_resident_ks() {
+  static std::mutex mu;
+  static int cache[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 512;
+  std::lock_guard<std::mutex> g(mu);
+  if (!cache[dev]) {
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_screen<KS>),
+                                                     64 * screen_waves<KS>(), 0) != hipSuccess || per <= 0)
+      per = 1;
+    cache[dev] = cus * per;
+  }
+  return (uint64_t)cache[dev];
+}
+
+uint64_t screen_resident(int ks) {
+  return ks == 8 ? screen_resident_ks<8>() : ks == 16 ? screen_resident_ks<16>() : screen_resident_ks<32>();
+}
+
+int screen_ks(uint64_t d) { return d <= 16 ? 8 : d <= 32 ? 16 : 32; }
+
+// Point slices per query block (the SW QT query tiles of a workgroup).  Each
+// workgroup walks its slice's tiles, so a launch takes ceil(workgroups /
+// resident) rounds of about ceil(tiles / slices) + W0 tile steps, W0 = 256 the
+// per-workgroup cost of filling its lane lists (a short slice keeps the lists'
+// thresholds high: more appends and merges).  The count minimises that
+// (fewest slices among equals), between at least 8 tiles per slice and, for
+// k > KP, at least k / 4 interleaved slices: a query's top k then spread over
+// at least k / 2 lists of KP, about two per list, so a list rarely fills below
+// tau_A (the completeness check flags the query when one does).  Measured on
+// 2^20 64-d points (k_screen ms by slice count): 20,000 queries at k = 32,
+// 8: 7.19, 12: 5.67, 19: 5.74, 32: 6.23, 64: 7.49, 512: 15.15; 8,192 queries
+// at k = 10, 8: 3.36, 16: 2.26, 32: 2.55; 1,024 queries, 128: 0.54, 256: 0.63.
+// Returns the slice count; *per = tiles per slice.
 uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
-  const uint64_t qtiles = (qp.nq + 31) / 32;
-  const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
-  uint64_t slices = std::max<uint64_t>(1, (4096 + qtiles - 1) / qtiles);
-  if (qp.k > (uint64_t)KP) slices = std::max<uint64_t>(slices, (qp.k + 3) / 4);
-  slices = std::min<uint64_t>({slices, MAX_SLICES, std::max<uint32_t>(1, n_tiles / 8)});
-  *per = (uint32_t)((n_tiles + slices - 1) / slices);
-  return (n_tiles + *per - 1) / *per;
+  const int ks = screen_ks(t.d);
+  const uint64_t block = (uint64_t)32 * (ks == 8 ? screen_waves<8>() * screen_qt<8>()
+                                                 : ks == 16 ? screen_waves<16>() * screen_qt<16>()
+                                                            : screen_waves<32>() * screen_qt<32>());
+  const uint64_t gx = (qp.nq + block - 1) / block;
+  const uint64_t n_tiles = (t.n + 31) / 32;
+  const uint64_t resident = screen_resident(ks);
+  const uint64_t hi = std::min<uint64_t>(MAX_SLICES, std::max<uint64_t>(1, n_tiles / 8));
+  const uint64_t lo = std::min<uint64_t>(hi, qp.k > (uint64_t)KP ? (qp.k + 3) / 4 : 1);
+  uint64_t best_s = lo, best_cost = UINT64_MAX, best_per = (n_tiles + lo - 1) / lo;
+  for (uint64_t S = lo; S <= hi; ++S) {
+    const uint64_t p = (n_tiles + S - 1) / S;
+    const uint64_t used = (n_tiles + p - 1) / p;  // slices that hold tiles
+    const uint64_t cost = (gx * used + resident - 1) / resident * (p + 256);
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_s = used;
+      best_per = p;
+    }
+  }
+  *per = (uint32_t)best_per;
+  return best_s;
 }
 
 }  // namespace
@@ -770,8 +860,9 @@ size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp) {
   uint32_t per = 0;
   const uint64_t slices = screen_slices(t, qp, &per);
   const uint64_t nq = qp.nq;
+  const uint64_t ks = (uint64_t)screen_ks(t.d);
   return 2 * scratch_piece(nq * 2 * slices * KP * 4) + scratch_piece(nq * 4) + scratch_piece(nq * CAND_CAP * 4) +
-         scratch_piece(nq) + scratch_piece(4);
+         scratch_piece(nq) + scratch_piece(4) + scratch_piece(nq * 2 * ks * 2) + scratch_piece(nq * 8);
 }
 
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
@@ -796,15 +887,26 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   if (e == hipSuccess) e = sc.get(&cand, nq * CAND_CAP);
   if (e == hipSuccess) e = sc.get(&flag, nq);
   if (e == hipSuccess) e = sc.get(&nflag, 1);
+  const int ks = screen_ks(t.d);
+  uint16_t* qrows = nullptr;
+  uint2* qnorm = nullptr;
+  if (e == hipSuccess) e = sc.get(&qrows, nq * 2 * ks);
+  if (e == hipSuccess) e = sc.get(&qnorm, nq);
   if (e != hipSuccess) {
     msg = std::string("screen scratch: ") + hipGetErrorString(e);
     return 100;
   }
+  {
+    const uint32_t pb = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 4096);
+    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, (uint32_t)t.d, t.frame, qrows, qnorm);
+    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, (uint32_t)t.d, t.frame, qrows, qnorm);
+    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, (uint32_t)t.d, t.frame, qrows, qnorm);
+  }
   SArgs sa{};
   sa.rows_h = t.rows_h;
   sa.pnorm_h = t.pnorm_h;
-  sa.frame = t.frame;
-  sa.q = trav.q;
+  sa.qrows = reinterpret_cast<const uint4*>(qrows);
+  sa.qnorm = qnorm;
   sa.part_a = part_a;
   sa.part_i = part_i;
   sa.nq = nq;
@@ -814,9 +916,9 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   sa.tiles_per_slice = per;
   sa.interleave = k > (uint32_t)KP;
   sa.slices = (uint32_t)slices;
-  const int ks = t.d <= 16 ? 8 : t.d <= 32 ? 16 : 32;
   const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
-  const dim3 grid((uint32_t)((qtiles + sw - 1) / sw), (uint32_t)slices);
+  const int qt = ks == 8 ? screen_qt<8>() : ks == 16 ? screen_qt<16>() : screen_qt<32>();
+  const dim3 grid((uint32_t)((qtiles + (uint64_t)sw * qt - 1) / ((uint64_t)sw * qt)), (uint32_t)slices);
   {
     // Algorithmic work: 2 D FLOPs per (query, point) pair of the contraction.
     KernelTimer tm(s);

@@ -95,21 +95,12 @@ struct SArgs {
   const uint2* __restrict__ qnorm;      // [nq] their norm-block words, query side
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
   uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
+  const float* __restrict__ thr0;  // [nq] starting threshold per query (k_screen_seed), or null: +inf
   uint64_t nq;
   uint32_t D, n, n_tiles, slices;
-  uint32_t tiles_per_slice;  // contiguous slices (k <= KP)
-  uint32_t interleave;       // k > KP: slice y takes tiles y, y + slices, ...
+  // Slice y takes point tiles y tb_mul + i ts for i < tmax (below n_tiles).
+  uint32_t tb_mul, ts, tmax;
 };
-
-// Query tiles (32 queries each) per wave.  Two would share each point
-// operand read from LDS between two independent accumulator chains, but their
-// registers (186 VGPRs) leave 2 waves per SIMD instead of 4: 64-d rows, 2^20
-// points, 65,536 queries 19.1 vs 13.5 ms with one tile.
-#ifndef KDT_SCREEN_QT32
-#define KDT_SCREEN_QT32 1
-#endif
-template <int KS>
-constexpr int screen_qt() { return KS <= 16 ? 1 : KDT_SCREEN_QT32; }
 
 // Waves per workgroup sharing each point tile.  Rows of 16 and 32 columns: 4
 // (the per-tile barrier waits on fewer waves; 16-d uniform rows 1.8 vs 2.3 ms,
@@ -117,7 +108,7 @@ constexpr int screen_qt() { return KS <= 16 ? 1 : KDT_SCREEN_QT32; }
 // normal rows at k = 10 3% slower).  Rows of 64 columns: 8 query tiles per
 // workgroup (with 4, each point tile is read twice as often from L2).
 #ifndef KDT_SCREEN_SW32
-#define KDT_SCREEN_SW32 (8 / KDT_SCREEN_QT32)
+#define KDT_SCREEN_SW32 8
 #endif
 template <int KS>
 constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
@@ -227,7 +218,7 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, co
 #endif
 template <int KS>
 constexpr uint32_t insert_slots() { return KS <= 8 ? KDT_SCREEN_IB8 : KS <= 16 ? 12 : KDT_SCREEN_IB32; }
-constexpr int CHK = 8;
+constexpr int CHK = 8;  // (Screen::tile checks after every two groups of four rows)
 static_assert(KDT_SCREEN_IB8 > CHK && KDT_SCREEN_IB32 > CHK, "a buffer holds more than one check interval");
 
 // The queries in the screen's operand form, once per call (every point
@@ -252,94 +243,99 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
   }
 }
 
-// One wave's state for its QT query tiles: per tile the query operands, the
-// lane's list and its insert buffer (tile u, slot e at buf[(u IB + e) 64 SW]).
+// One wave's state: the query operands, the lane's list and its insert buffer
+// (slot e at buf[e 64 SW]).  A tile's 16 values per lane are tested in four
+// groups of four rows: a group's per-element tests run only when some lane of
+// the wave has a value in it under its threshold (branch-free minima first).
 template <int KS>
 struct Screen {
   static constexpr int NS = KS / 8;  // f16x8 operands per lane half
   static constexpr int RU = KS / 4, RS = RU + 1;
-  static constexpr int QT = screen_qt<KS>();
   static constexpr int SW = screen_waves<KS>();
   static constexpr uint32_t IB = insert_slots<KS>();
-  f16x8 qh[QT][NS];     // -2 q^
-  bf16x8 qnb[QT];       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
-  float la[QT][KP];     // the KP smallest A seen by this lane, ascending
-  uint32_t li[QT][KP];  // their tree positions
-  float thr[QT];        // la[u][KP - 1] at the last merge
-  uint32_t cnt[QT];     // entries in this lane's buffer
+  f16x8 qh[NS];     // -2 q^
+  bf16x8 qnb;       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
+  float la[KP];     // the KP smallest A seen by this lane, ascending
+  uint32_t li[KP];  // their tree positions
+  float thr;        // min(thr0, la[KP - 1] at the last merge)
+  float thr0;       // the query's starting threshold (+inf without a seed)
+  uint32_t cnt;     // entries in this lane's buffer
   uint2* buf;
 
-  __device__ __forceinline__ void insert(int u, float A, uint32_t row) {  // dropping the largest
+  __device__ __forceinline__ void insert(float A, uint32_t row) {  // dropping the largest
 #pragma unroll
     for (int e = KP - 1; e > 0; --e) {
-      const bool up = A < la[u][e - 1];
-      const bool here = !up && A < la[u][e];
-      la[u][e] = up ? la[u][e - 1] : (here ? A : la[u][e]);
-      li[u][e] = up ? li[u][e - 1] : (here ? row : li[u][e]);
+      const bool up = A < la[e - 1];
+      const bool here = !up && A < la[e];
+      la[e] = up ? la[e - 1] : (here ? A : la[e]);
+      li[e] = up ? li[e - 1] : (here ? row : li[e]);
     }
-    if (A < la[u][0]) {
-      la[u][0] = A;
-      li[u][0] = row;
+    if (A < la[0]) {
+      la[0] = A;
+      li[0] = row;
     }
   }
 
-  // Merge every lane's buffer of tile u into its list; the whole wave calls it.
-  __device__ __forceinline__ void merge(int u) {
+  // Merge every lane's buffer into its list; the whole wave calls it.
+  __device__ __forceinline__ void merge() {
 #pragma unroll 1
     for (uint32_t e = 0; e < IB; ++e) {
-      if (!__builtin_amdgcn_ballot_w64(e < cnt[u])) break;
-      if (e < cnt[u]) {
-        const uint2 v = buf[(u * IB + e) * 64 * SW];
+      if (!__builtin_amdgcn_ballot_w64(e < cnt)) break;
+      if (e < cnt) {
+        const uint2 v = buf[e * 64 * SW];
         const float A = __uint_as_float(v.x);
-        if (A < la[u][KP - 1]) insert(u, A, v.y);
+        if (A < la[KP - 1]) insert(A, v.y);
       }
     }
-    thr[u] = la[u][KP - 1];
-    cnt[u] = 0;
+    thr = fminf(thr0, la[KP - 1]);
+    cnt = 0;
   }
 
-  // The QT x 32 queries of the wave against point tile t (32 rows in LDS).
-  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
+  // The wave's 32 queries against the 32 point rows in LDS: -2 q^.p^ from
+  // the NS fp16 products, then the norms.
+  __device__ __forceinline__ floatx16 contract(uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) const {
     const uint4* rb = rows + j * RS + h * NS;
     // Norm block, point side, of row j in half 0 (built with the rows).
     // Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
     const uint2 pw = pns[j];
     const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw.x, pw.y, 0u) : make_uint4(0u, 0u, 0u, 0u);
-    floatx16 acc[QT];
+    floatx16 acc = {};
 #pragma unroll
-    for (int u = 0; u < QT; ++u) acc[u] = floatx16{};
-    // Each point operand, read once, feeds the QT chains.
+    for (int v = 0; v < NS; ++v)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, rb[v]), qh[v], acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb, acc, 0, 0, 0);
+  }
+
+  // Contract point tile t and test its values.  Element r of the accumulator
+  // is row rowbase | (8 (r / 4) + r % 4) (disjoint bits, one immediate OR).
+  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
+    const floatx16 acc = contract(j, h, rows, pns);
+    float g[4];
 #pragma unroll
-    for (int v = 0; v < NS; ++v) {
-      const f16x8 pv = __builtin_bit_cast(f16x8, rb[v]);
-#pragma unroll
-      for (int u = 0; u < QT; ++u) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pv, qh[u][v], acc[u], 0, 0, 0);
-    }
-#pragma unroll
-    for (int u = 0; u < QT; ++u)
-      acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb[u], acc[u], 0, 0, 0);
-    // Element r is row rowbase | (8 (r / 4) + r % 4): disjoint bits, one
-    // immediate OR, nothing per element to keep in registers across tiles.
+    for (int q = 0; q < 4; ++q) g[q] = fminf(fminf(acc[4 * q], acc[4 * q + 1]), fminf(acc[4 * q + 2], acc[4 * q + 3]));
+    if (!__builtin_amdgcn_ballot_w64(fminf(fminf(g[0], g[1]), fminf(g[2], g[3])) < thr)) return;
     const uint32_t rowbase = t * 32 + 4 * h;
 #pragma unroll
-    for (int u = 0; u < QT; ++u) {
+    for (int q = 0; q < 4; ++q) {
+      if (__builtin_amdgcn_ballot_w64(g[q] < thr)) {
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const float A = acc[u][r];
-        // A wave-uniform branch round the append: a plain divergent `if` here
-        // runs the append under an empty exec mask on every element instead of
-        // skipping it (the barrier keeps the two tests from being merged).
-        if (__builtin_amdgcn_ballot_w64(A < thr[u])) {
-          __builtin_amdgcn_sched_barrier(0);
-          if (A < thr[u]) {
-            const uint32_t row = rowbase | (8 * (r / 4) + (r % 4));
-            buf[(u * IB + cnt[u]) * 64 * SW] = make_uint2(__float_as_uint(A), row);
-            ++cnt[u];
+        for (int r = 4 * q; r < 4 * q + 4; ++r) {
+          const float A = acc[r];
+          // A wave-uniform branch round the append: a plain divergent `if`
+          // here runs the append under an empty exec mask on every element
+          // instead of skipping it (the barrier keeps the two tests apart).
+          if (__builtin_amdgcn_ballot_w64(A < thr)) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (A < thr) {
+              const uint32_t row = rowbase | (8 * (r / 4) + (r % 4));
+              buf[cnt * 64 * SW] = make_uint2(__float_as_uint(A), row);
+              ++cnt;
+            }
           }
         }
-        // cnt <= IB - CHK before each group of CHK elements, so <= IB after it.
-        if (r % CHK == CHK - 1 && __builtin_amdgcn_ballot_w64(cnt[u] > IB - CHK)) merge(u);
       }
+      // cnt <= IB - CHK before each pair of groups (CHK = 8 elements), so <= IB after it.
+      if (q % 2 == 1 && __builtin_amdgcn_ballot_w64(cnt > IB - CHK)) merge();
     }
   }
 };
@@ -351,31 +347,30 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int TS = Stage<KS>::TS;
   __shared__ uint4 s_rows[2][TS * 32 * RS];
   __shared__ uint2 s_pn[2][TS * 32];
-  constexpr int QT = screen_qt<KS>();
-  __shared__ uint2 s_buf[QT * insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
+  __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
-  // Wave w of the workgroup takes query tiles (blockIdx.x SW + w) QT + u.
-  const uint64_t qtile0 = ((uint64_t)blockIdx.x * SW + (tid >> 6)) * QT;
-  const bool active = qtile0 * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
+  // Wave w of the workgroup takes query tile blockIdx.x SW + w.
+  const uint64_t qtile = (uint64_t)blockIdx.x * SW + (tid >> 6);
+  const bool active = qtile * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
   Screen<KS> sc;
   constexpr int NS = KS / 8, RQ = KS / 4;
-#pragma unroll
-  for (int u = 0; u < QT; ++u) {
+  {
     // Rows past nq read the last query (their lists are not written).
-    const uint64_t qrow = min((qtile0 + u) * 32 + j, a.nq - 1);
+    const uint64_t qrow = min(qtile * 32 + j, a.nq - 1);
 #pragma unroll
-    for (int v = 0; v < NS; ++v) sc.qh[u][v] = __builtin_bit_cast(f16x8, a.qrows[qrow * RQ + h * NS + v]);
+    for (int v = 0; v < NS; ++v) sc.qh[v] = __builtin_bit_cast(f16x8, a.qrows[qrow * RQ + h * NS + v]);
     const uint2 w = a.qnorm[qrow];
-    sc.qnb[u] = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
-#pragma unroll
-    for (int e = 0; e < KP; ++e) {
-      sc.la[u][e] = INFINITY;
-      sc.li[u][e] = 0xFFFFFFFFu;
-    }
-    sc.thr[u] = INFINITY;
-    sc.cnt[u] = 0;
+    sc.qnb = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
+    sc.thr0 = a.thr0 ? a.thr0[qrow] : INFINITY;
   }
+#pragma unroll
+  for (int e = 0; e < KP; ++e) {
+    sc.la[e] = INFINITY;
+    sc.li[e] = 0xFFFFFFFFu;
+  }
+  sc.thr = sc.thr0;
+  sc.cnt = 0;
   sc.buf = s_buf + tid;
 
   // Local tile i of slice y is global point tile tb + i ts.  The rows are in
@@ -385,18 +380,11 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   // (k > KP: every S-th tile, tb = y, ts = S, so that each slice samples all of
   // space; 16-d uniform data, k = 32: 12-28% of the queries overflowed a list
   // and took the traversal with contiguous slices, none interleaved).  The
-  // loop below runs over local indices.
-  const uint32_t S = a.slices, sy = blockIdx.y;
-  uint32_t tb, ts, t1;
-  if (a.interleave) {
-    tb = sy;
-    ts = S;
-    t1 = sy < a.n_tiles ? (a.n_tiles - sy + S - 1) / S : 0;
-  } else {
-    tb = sy * a.tiles_per_slice;
-    ts = 1;
-    t1 = tb < a.n_tiles ? min(a.tiles_per_slice, a.n_tiles - tb) : 0;
-  }
+  // seeding pass (k_screen_seed) takes every G-th tile: tb = y G, ts = G S.
+  // The loop below runs over local indices.
+  const uint32_t sy = blockIdx.y;
+  const uint32_t tb = sy * a.tb_mul, ts = a.ts;
+  const uint32_t t1 = tb < a.n_tiles ? min(a.tmax, (a.n_tiles - tb + ts - 1) / ts) : 0;
 #define KDT_TILE(i) (tb + (i) * ts)
   // Steps of TS local tiles: step q holds local tiles q TS .. q TS + TS - 1
   // (those from t1 on are re-reads of tile t1 - 1 and are not contracted).
@@ -438,17 +426,14 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   }
 #undef KDT_SCREEN_STEP
 #undef KDT_TILE
+  if (active) sc.merge();  // wave-uniform
+  const uint64_t qrow = qtile * 32 + j;
+  if (qrow < a.nq) {
+    const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
 #pragma unroll
-  for (int u = 0; u < QT; ++u) {
-    if (active) sc.merge(u);  // wave-uniform
-    const uint64_t qrow = (qtile0 + u) * 32 + j;
-    if (qrow < a.nq) {
-      const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
-#pragma unroll
-      for (int e = 0; e < KP; ++e) {
-        a.part_a[base + e] = sc.la[u][e];
-        a.part_i[base + e] = sc.li[u][e];
-      }
+    for (int e = 0; e < KP; ++e) {
+      a.part_a[base + e] = sc.la[e];
+      a.part_i[base + e] = sc.li[e];
     }
   }
 }
@@ -461,6 +446,7 @@ struct SelArgs {
   uint32_t* __restrict__ cand;
   uint8_t* __restrict__ flag;
   uint32_t* __restrict__ nflag;  // queries flagged so far (cleared before the select)
+  float* __restrict__ seed;      // seeding mode: [nq] tau_A of the lists' k-th A, and nothing else
   uint64_t nq;
   const double* __restrict__ frame;  // [D + 1] the tree's centre and sigma
   uint32_t D, k, lists;
@@ -543,6 +529,13 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
     const double t = (r * r + E2) * (1.0 + 1e-12);
     ta = (float)t;
     if ((double)ta < t) ta = nextafterf(ta, INFINITY);
+  }
+  if (a.seed) {
+    // Any k points' A values bound A_k from above, so tau_A of the sampled
+    // lists' k-th value is >= the final tau_A: a lane may drop every value
+    // above it (no candidate is lost).  Out-of-range queries: no seed.
+    if (lane == 0) a.seed[qi] = qmax * sg <= 2048.0 ? ta : INFINITY;
+    return;
   }
   // Completeness: a full list whose last value is <= tau_A may have dropped a
   // candidate.  Unfilled slots (A = +inf, position 0xFFFFFFFF) end a list.
@@ -810,7 +803,7 @@ uint64_t screen_resident(int ks) {
 
 int screen_ks(uint64_t d) { return d <= 16 ? 8 : d <= 32 ? 16 : 32; }
 
-// Point slices per query block (the SW QT query tiles of a workgroup).  Each
+// Point slices per query block (the SW query tiles of a workgroup).  Each
 // workgroup walks its slice's tiles, so a launch takes ceil(workgroups /
 // resident) rounds of about ceil(tiles / slices) + W0 tile steps, W0 = 256 the
 // per-workgroup cost of filling its lane lists (a short slice keeps the lists'
@@ -825,9 +818,7 @@ int screen_ks(uint64_t d) { return d <= 16 ? 8 : d <= 32 ? 16 : 32; }
 // Returns the slice count; *per = tiles per slice.
 uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
   const int ks = screen_ks(t.d);
-  const uint64_t block = (uint64_t)32 * (ks == 8 ? screen_waves<8>() * screen_qt<8>()
-                                                 : ks == 16 ? screen_waves<16>() * screen_qt<16>()
-                                                            : screen_waves<32>() * screen_qt<32>());
+  const uint64_t block = (uint64_t)32 * (ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>());
   const uint64_t gx = (qp.nq + block - 1) / block;
   const uint64_t n_tiles = (t.n + 31) / 32;
   const uint64_t resident = screen_resident(ks);
@@ -848,6 +839,18 @@ uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
   return best_s;
 }
 
+// Threshold seeding.  A lane's list starts empty, so its threshold is +inf
+// until 16 values have passed and then falls slowly: over a slice of m values
+// a lane keeps about 16 (1 + ln(m / 16)) of them, each an LDS append and a
+// merge through the insert network, most of them early.  A seeding pass over
+// every SEED_STRIDE-th point tile gives each query an upper bound of A_k (the
+// k-th A among the sampled points) and from it a threshold that the full pass
+// starts from in every lane.  Trees of fewer than SEED_MIN_TILES tiles are
+// screened without it.
+constexpr uint32_t SEED_STRIDE = 64;
+constexpr uint32_t SEED_MIN_TILES = 1024;
+bool screen_seeded(uint32_t n_tiles) { return n_tiles >= SEED_MIN_TILES; }
+
 }  // namespace
 
 bool screen_applies(const Tree& t, const QueryParams& qp) {
@@ -861,8 +864,10 @@ size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp) {
   const uint64_t slices = screen_slices(t, qp, &per);
   const uint64_t nq = qp.nq;
   const uint64_t ks = (uint64_t)screen_ks(t.d);
+  const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
   return 2 * scratch_piece(nq * 2 * slices * KP * 4) + scratch_piece(nq * 4) + scratch_piece(nq * CAND_CAP * 4) +
-         scratch_piece(nq) + scratch_piece(4) + scratch_piece(nq * 2 * ks * 2) + scratch_piece(nq * 8);
+         scratch_piece(nq) + scratch_piece(4) + scratch_piece(nq * 2 * ks * 2) + scratch_piece(nq * 8) +
+         (screen_seeded(n_tiles) ? scratch_piece(nq * 4) : 0);
 }
 
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
@@ -892,6 +897,9 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   uint2* qnorm = nullptr;
   if (e == hipSuccess) e = sc.get(&qrows, nq * 2 * ks);
   if (e == hipSuccess) e = sc.get(&qnorm, nq);
+  const bool seeded = screen_seeded(n_tiles);
+  float* thr0 = nullptr;
+  if (e == hipSuccess && seeded) e = sc.get(&thr0, nq);
   if (e != hipSuccess) {
     msg = std::string("screen scratch: ") + hipGetErrorString(e);
     return 100;
@@ -913,21 +921,25 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   sa.D = (uint32_t)t.d;
   sa.n = (uint32_t)t.n;
   sa.n_tiles = n_tiles;
-  sa.tiles_per_slice = per;
-  sa.interleave = k > (uint32_t)KP;
   sa.slices = (uint32_t)slices;
-  const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
-  const int qt = ks == 8 ? screen_qt<8>() : ks == 16 ? screen_qt<16>() : screen_qt<32>();
-  const dim3 grid((uint32_t)((qtiles + (uint64_t)sw * qt - 1) / ((uint64_t)sw * qt)), (uint32_t)slices);
-  {
-    // Algorithmic work: 2 D FLOPs per (query, point) pair of the contraction.
-    KernelTimer tm(s);
-    if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(sa);
-    else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(sa);
-    else k_screen<32><<<grid, 64 * sw, 0, s>>>(sa);
-    tm.stop(ks == 8 ? "k_screen<8>" : ks == 16 ? "k_screen<16>" : "k_screen<32>",
-            2.0 * (double)t.d * (double)t.n * (double)nq);
+  if (k > (uint32_t)KP) {  // interleaved
+    sa.tb_mul = 1;
+    sa.ts = (uint32_t)slices;
+    sa.tmax = 0xFFFFFFFFu;
+  } else {  // contiguous
+    sa.tb_mul = per;
+    sa.ts = 1;
+    sa.tmax = per;
   }
+  const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
+  auto screen = [&](const SArgs& args, uint32_t ny, const char* name, double work) {
+    const dim3 grid((uint32_t)((qtiles + sw - 1) / sw), ny);
+    KernelTimer tm(s);
+    if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(args);
+    else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(args);
+    else k_screen<32><<<grid, 64 * sw, 0, s>>>(args);
+    tm.stop(name, work);
+  };
 
   SelArgs se{};
   se.part_a = part_a;
@@ -945,6 +957,30 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   se.sigma = t.scr_sigma;
   se.pc = t.scr_pc;
   se.cutoff = qp.cutoff;
+
+  if (seeded) {
+    // Seeding pass: every SEED_STRIDE-th point tile, in S0 slices, then each
+    // query's tau_A of the k-th A over those lists as every lane's starting
+    // threshold in the full pass (the lists' scratch is reused).
+    const uint32_t ns = (n_tiles + SEED_STRIDE - 1) / SEED_STRIDE;
+    const uint32_t s0 = (uint32_t)std::min<uint64_t>(slices, std::max<uint32_t>(1, ns / 8));
+    SArgs sp = sa;
+    sp.thr0 = nullptr;
+    sp.slices = s0;
+    sp.tb_mul = SEED_STRIDE;
+    sp.ts = SEED_STRIDE * s0;
+    sp.tmax = 0xFFFFFFFFu;
+    screen(sp, s0, "k_screen_seed", 2.0 * (double)t.d * 32.0 * (double)ns * (double)nq);
+    SelArgs ss = se;
+    ss.lists = 2 * s0;
+    ss.seed = thr0;
+    k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(ss);
+    sa.thr0 = thr0;
+  }
+  // Algorithmic work: 2 D FLOPs per (query, point) pair of the contraction.
+  screen(sa, (uint32_t)slices, ks == 8 ? "k_screen<8>" : ks == 16 ? "k_screen<16>" : "k_screen<32>",
+         2.0 * (double)t.d * (double)t.n * (double)nq);
+
   launch_zero_u32(nflag, 1, s);
   {
     KernelTimer tm(s);

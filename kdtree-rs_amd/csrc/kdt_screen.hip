@@ -102,13 +102,22 @@ struct SArgs {
   uint32_t tb_mul, ts, tmax;
 };
 
+// Query tiles (32 queries each) per wave: with two, each point operand read
+// from LDS feeds two accumulator chains (half the LDS traffic per matrix
+// instruction) at the cost of the registers of a second query tile.
+#ifndef KDT_SCREEN_QT32
+#define KDT_SCREEN_QT32 1
+#endif
+template <int KS>
+constexpr int screen_qt() { return KS <= 16 ? 1 : KDT_SCREEN_QT32; }
+
 // Waves per workgroup sharing each point tile.  Rows of 16 and 32 columns: 4
 // (the per-tile barrier waits on fewer waves; 16-d uniform rows 1.8 vs 2.3 ms,
 // 16-d normal rows 3% faster; 24-d uniform rows at k = 32 5.1 vs 6.2 ms, 32-d
 // normal rows at k = 10 3% slower).  Rows of 64 columns: 8 query tiles per
 // workgroup (with 4, each point tile is read twice as often from L2).
 #ifndef KDT_SCREEN_SW32
-#define KDT_SCREEN_SW32 8
+#define KDT_SCREEN_SW32 (8 / KDT_SCREEN_QT32)
 #endif
 template <int KS>
 constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
@@ -243,100 +252,110 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
   }
 }
 
-// One wave's state: the query operands, the lane's list and its insert buffer
-// (slot e at buf[e 64 SW]).  A tile's 16 values per lane are tested in four
-// groups of four rows: a group's per-element tests run only when some lane of
-// the wave has a value in it under its threshold (branch-free minima first).
+// One wave's state for its QT query tiles: per tile the query operands, the
+// lane's list and its insert buffer (tile u, slot e at buf[(u IB + e) 64 SW]).
+// A tile's 16 values per lane are tested in four groups of four rows: a
+// group's per-element tests run only when some lane of the wave has a value
+// in it under its threshold (branch-free minima first).
 template <int KS>
 struct Screen {
   static constexpr int NS = KS / 8;  // f16x8 operands per lane half
   static constexpr int RU = KS / 4, RS = RU + 1;
+  static constexpr int QT = screen_qt<KS>();
   static constexpr int SW = screen_waves<KS>();
   static constexpr uint32_t IB = insert_slots<KS>();
-  f16x8 qh[NS];     // -2 q^
-  bf16x8 qnb;       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
-  float la[KP];     // the KP smallest A seen by this lane, ascending
-  uint32_t li[KP];  // their tree positions
-  float thr;        // min(thr0, la[KP - 1] at the last merge)
-  float thr0;       // the query's starting threshold (+inf without a seed)
-  uint32_t cnt;     // entries in this lane's buffer
+  f16x8 qh[QT][NS];     // -2 q^
+  bf16x8 qnb[QT];       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
+  float la[QT][KP];     // the KP smallest A seen by this lane, ascending
+  uint32_t li[QT][KP];  // their tree positions
+  float thr[QT];        // min(thr0, la[KP - 1] at the last merge)
+  float thr0[QT];       // the query's starting threshold (+inf without a seed)
+  uint32_t cnt[QT];     // entries in this lane's buffer
   uint2* buf;
 
-  __device__ __forceinline__ void insert(float A, uint32_t row) {  // dropping the largest
+  __device__ __forceinline__ void insert(int u, float A, uint32_t row) {  // dropping the largest
 #pragma unroll
     for (int e = KP - 1; e > 0; --e) {
-      const bool up = A < la[e - 1];
-      const bool here = !up && A < la[e];
-      la[e] = up ? la[e - 1] : (here ? A : la[e]);
-      li[e] = up ? li[e - 1] : (here ? row : li[e]);
+      const bool up = A < la[u][e - 1];
+      const bool here = !up && A < la[u][e];
+      la[u][e] = up ? la[u][e - 1] : (here ? A : la[u][e]);
+      li[u][e] = up ? li[u][e - 1] : (here ? row : li[u][e]);
     }
-    if (A < la[0]) {
-      la[0] = A;
-      li[0] = row;
+    if (A < la[u][0]) {
+      la[u][0] = A;
+      li[u][0] = row;
     }
   }
 
-  // Merge every lane's buffer into its list; the whole wave calls it.
-  __device__ __forceinline__ void merge() {
+  // Merge every lane's buffer of tile u into its list; the whole wave calls it.
+  __device__ __forceinline__ void merge(int u) {
 #pragma unroll 1
     for (uint32_t e = 0; e < IB; ++e) {
-      if (!__builtin_amdgcn_ballot_w64(e < cnt)) break;
-      if (e < cnt) {
-        const uint2 v = buf[e * 64 * SW];
+      if (!__builtin_amdgcn_ballot_w64(e < cnt[u])) break;
+      if (e < cnt[u]) {
+        const uint2 v = buf[(u * IB + e) * 64 * SW];
         const float A = __uint_as_float(v.x);
-        if (A < la[KP - 1]) insert(A, v.y);
+        if (A < la[u][KP - 1]) insert(u, A, v.y);
       }
     }
-    thr = fminf(thr0, la[KP - 1]);
-    cnt = 0;
+    thr[u] = fminf(thr0[u], la[u][KP - 1]);
+    cnt[u] = 0;
   }
 
-  // The wave's 32 queries against the 32 point rows in LDS: -2 q^.p^ from
-  // the NS fp16 products, then the norms.
-  __device__ __forceinline__ floatx16 contract(uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) const {
-    const uint4* rb = rows + j * RS + h * NS;
-    // Norm block, point side, of row j in half 0 (built with the rows).
-    // Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
-    const uint2 pw = pns[j];
-    const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw.x, pw.y, 0u) : make_uint4(0u, 0u, 0u, 0u);
-    floatx16 acc = {};
-#pragma unroll
-    for (int v = 0; v < NS; ++v)
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, rb[v]), qh[v], acc, 0, 0, 0);
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb, acc, 0, 0, 0);
-  }
-
-  // Contract point tile t and test its values.  Element r of the accumulator
-  // is row rowbase | (8 (r / 4) + r % 4) (disjoint bits, one immediate OR).
-  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
-    const floatx16 acc = contract(j, h, rows, pns);
+  // Test the 16 values of query tile u.  Element r of the accumulator is row
+  // rowbase | (8 (r / 4) + r % 4) (disjoint bits, one immediate OR).
+  __device__ __forceinline__ void test(int u, const floatx16& acc, uint32_t rowbase) {
     float g[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) g[q] = fminf(fminf(acc[4 * q], acc[4 * q + 1]), fminf(acc[4 * q + 2], acc[4 * q + 3]));
-    if (!__builtin_amdgcn_ballot_w64(fminf(fminf(g[0], g[1]), fminf(g[2], g[3])) < thr)) return;
-    const uint32_t rowbase = t * 32 + 4 * h;
+    if (!__builtin_amdgcn_ballot_w64(fminf(fminf(g[0], g[1]), fminf(g[2], g[3])) < thr[u])) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (__builtin_amdgcn_ballot_w64(g[q] < thr)) {
+      if (__builtin_amdgcn_ballot_w64(g[q] < thr[u])) {
 #pragma unroll
         for (int r = 4 * q; r < 4 * q + 4; ++r) {
           const float A = acc[r];
           // A wave-uniform branch round the append: a plain divergent `if`
           // here runs the append under an empty exec mask on every element
           // instead of skipping it (the barrier keeps the two tests apart).
-          if (__builtin_amdgcn_ballot_w64(A < thr)) {
+          if (__builtin_amdgcn_ballot_w64(A < thr[u])) {
             __builtin_amdgcn_sched_barrier(0);
-            if (A < thr) {
+            if (A < thr[u]) {
               const uint32_t row = rowbase | (8 * (r / 4) + (r % 4));
-              buf[cnt * 64 * SW] = make_uint2(__float_as_uint(A), row);
-              ++cnt;
+              buf[(u * IB + cnt[u]) * 64 * SW] = make_uint2(__float_as_uint(A), row);
+              ++cnt[u];
             }
           }
         }
       }
       // cnt <= IB - CHK before each pair of groups (CHK = 8 elements), so <= IB after it.
-      if (q % 2 == 1 && __builtin_amdgcn_ballot_w64(cnt > IB - CHK)) merge();
+      if (q % 2 == 1 && __builtin_amdgcn_ballot_w64(cnt[u] > IB - CHK)) merge(u);
     }
+  }
+
+  // The wave's QT x 32 queries against point tile t (32 rows in LDS): -2 q^.p^
+  // from the NS fp16 products (each point operand, read once, feeds the QT
+  // chains), then the norms, then the tests.
+  __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
+    const uint4* rb = rows + j * RS + h * NS;
+    // Norm block, point side, of row j in half 0 (built with the rows).
+    // Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
+    const uint2 pw = pns[j];
+    const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw.x, pw.y, 0u) : make_uint4(0u, 0u, 0u, 0u);
+    floatx16 acc[QT];
+#pragma unroll
+    for (int u = 0; u < QT; ++u) acc[u] = floatx16{};
+#pragma unroll
+    for (int v = 0; v < NS; ++v) {
+      const f16x8 pv = __builtin_bit_cast(f16x8, rb[v]);
+#pragma unroll
+      for (int u = 0; u < QT; ++u) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pv, qh[u][v], acc[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < QT; ++u)
+      acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, pnw), qnb[u], acc[u], 0, 0, 0);
+#pragma unroll
+    for (int u = 0; u < QT; ++u) test(u, acc[u], t * 32 + 4 * h);
   }
 };
 
@@ -347,30 +366,32 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int TS = Stage<KS>::TS;
   __shared__ uint4 s_rows[2][TS * 32 * RS];
   __shared__ uint2 s_pn[2][TS * 32];
-  __shared__ uint2 s_buf[insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
+  constexpr int QT = screen_qt<KS>();
+  __shared__ uint2 s_buf[QT * insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t j = lane & 31, h = lane >> 5;
-  // Wave w of the workgroup takes query tile blockIdx.x SW + w.
-  const uint64_t qtile = (uint64_t)blockIdx.x * SW + (tid >> 6);
-  const bool active = qtile * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
+  // Wave w of the workgroup takes query tiles (blockIdx.x SW + w) QT + u.
+  const uint64_t qtile0 = ((uint64_t)blockIdx.x * SW + (tid >> 6)) * QT;
+  const bool active = qtile0 * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
   Screen<KS> sc;
   constexpr int NS = KS / 8, RQ = KS / 4;
-  {
+#pragma unroll
+  for (int u = 0; u < QT; ++u) {
     // Rows past nq read the last query (their lists are not written).
-    const uint64_t qrow = min(qtile * 32 + j, a.nq - 1);
+    const uint64_t qrow = min((qtile0 + u) * 32 + j, a.nq - 1);
 #pragma unroll
-    for (int v = 0; v < NS; ++v) sc.qh[v] = __builtin_bit_cast(f16x8, a.qrows[qrow * RQ + h * NS + v]);
+    for (int v = 0; v < NS; ++v) sc.qh[u][v] = __builtin_bit_cast(f16x8, a.qrows[qrow * RQ + h * NS + v]);
     const uint2 w = a.qnorm[qrow];
-    sc.qnb = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
-    sc.thr0 = a.thr0 ? a.thr0[qrow] : INFINITY;
-  }
+    sc.qnb[u] = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
+    sc.thr0[u] = a.thr0 ? a.thr0[qrow] : INFINITY;
 #pragma unroll
-  for (int e = 0; e < KP; ++e) {
-    sc.la[e] = INFINITY;
-    sc.li[e] = 0xFFFFFFFFu;
+    for (int e = 0; e < KP; ++e) {
+      sc.la[u][e] = INFINITY;
+      sc.li[u][e] = 0xFFFFFFFFu;
+    }
+    sc.thr[u] = sc.thr0[u];
+    sc.cnt[u] = 0;
   }
-  sc.thr = sc.thr0;
-  sc.cnt = 0;
   sc.buf = s_buf + tid;
 
   // Local tile i of slice y is global point tile tb + i ts.  The rows are in
@@ -426,14 +447,17 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   }
 #undef KDT_SCREEN_STEP
 #undef KDT_TILE
-  if (active) sc.merge();  // wave-uniform
-  const uint64_t qrow = qtile * 32 + j;
-  if (qrow < a.nq) {
-    const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
 #pragma unroll
-    for (int e = 0; e < KP; ++e) {
-      a.part_a[base + e] = sc.la[e];
-      a.part_i[base + e] = sc.li[e];
+  for (int u = 0; u < QT; ++u) {
+    if (active) sc.merge(u);  // wave-uniform
+    const uint64_t qrow = (qtile0 + u) * 32 + j;
+    if (qrow < a.nq) {
+      const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
+#pragma unroll
+      for (int e = 0; e < KP; ++e) {
+        a.part_a[base + e] = sc.la[u][e];
+        a.part_i[base + e] = sc.li[u][e];
+      }
     }
   }
 }
@@ -818,7 +842,9 @@ int screen_ks(uint64_t d) { return d <= 16 ? 8 : d <= 32 ? 16 : 32; }
 // Returns the slice count; *per = tiles per slice.
 uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
   const int ks = screen_ks(t.d);
-  const uint64_t block = (uint64_t)32 * (ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>());
+  const uint64_t block = (uint64_t)32 * (ks == 8 ? screen_waves<8>() * screen_qt<8>()
+                                                 : ks == 16 ? screen_waves<16>() * screen_qt<16>()
+                                                            : screen_waves<32>() * screen_qt<32>());
   const uint64_t gx = (qp.nq + block - 1) / block;
   const uint64_t n_tiles = (t.n + 31) / 32;
   const uint64_t resident = screen_resident(ks);
@@ -932,8 +958,9 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     sa.tmax = per;
   }
   const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
+  const int qt = ks == 8 ? screen_qt<8>() : ks == 16 ? screen_qt<16>() : screen_qt<32>();
   auto screen = [&](const SArgs& args, uint32_t ny, const char* name, double work) {
-    const dim3 grid((uint32_t)((qtiles + sw - 1) / sw), ny);
+    const dim3 grid((uint32_t)((qtiles + (uint64_t)sw * qt - 1) / ((uint64_t)sw * qt)), ny);
     KernelTimer tm(s);
     if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(args);
     else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(args);

@@ -873,8 +873,11 @@ uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
 // k-th A among the sampled points) and from it a threshold that the full pass
 // starts from in every lane.  Trees of fewer than SEED_MIN_TILES tiles are
 // screened without it.
+#ifndef KDT_SCREEN_SEED_MIN_TILES
+#define KDT_SCREEN_SEED_MIN_TILES 1024
+#endif
 constexpr uint32_t SEED_STRIDE = 64;
-constexpr uint32_t SEED_MIN_TILES = 1024;
+constexpr uint32_t SEED_MIN_TILES = KDT_SCREEN_SEED_MIN_TILES;
 bool screen_seeded(uint32_t n_tiles) { return n_tiles >= SEED_MIN_TILES; }
 
 }  // namespace

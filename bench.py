@@ -239,15 +239,19 @@ def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     nq = q.shape[0]
     od64, oi64 = F.query_device(tree, q, k, flags=0)
     od32, oi32 = F.query_device(tree, q, k, flags=F.KDT_QUERY_FP32_SCREEN)
+    odt, oit = F.query_device(tree, q, k, flags=F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_NO_TILE_WALK)
     F.timing_enable(True)
     F.timing_read()
     s64 = _time_steps(lambda: F.query_device(tree, q, k, od64, oi64, flags=0), steps, warmup)
     t64 = F.timing_read()
     s32 = _time_steps(lambda: F.query_device(tree, q, k, od32, oi32, flags=F.KDT_QUERY_FP32_SCREEN), steps, warmup)
     t32 = F.timing_read()
+    st = _time_steps(lambda: F.query_device(tree, q, k, odt, oit, flags=F.KDT_QUERY_FP32_SCREEN
+                                            | F.KDT_QUERY_NO_TILE_WALK), steps, warmup)
+    tt = F.timing_read()
     F.timing_enable(False)
-    same_idx = bool(torch.equal(oi64, oi32))
-    same_bytes = bool(torch.equal(od64, od32))
+    same_idx = bool(torch.equal(oi64, oi32)) and bool(torch.equal(oi64, oit))
+    same_bytes = bool(torch.equal(od64, od32)) and bool(torch.equal(od64, odt))
     d64, d32 = od64.cpu().numpy(), od32.cpu().numpy()
     rel = float(np.max(np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-300)))
     otree = OracleTree(data_h, ls)
@@ -268,7 +272,8 @@ def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     line = {
         "workload": "knn-gmm64-n4194304-d8-q1000000-k32-leafsize32-p2",
         "fp64": {"value": nq / s64, "ms_per_step": s64 * 1e3, "roofline": roof(t64)},
-        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3, "roofline": roof(t32)},
+        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3, "kernels": kernel_summary(t32)},
+        "fp32_screen_traversal": {"value": nq / st, "ms_per_step": st * 1e3, "roofline": roof(tt)},
         "unit": "queries/s", "steps": steps, "bytes_per_query": bq, "E": E, "V": V, "X": X,
         "distance_evals_per_s_fp64": E * nq / s64,
         "distance_flops_per_s_fp64": 3 * d * E * nq / s64,

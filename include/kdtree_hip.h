@@ -49,10 +49,13 @@ enum kdt_error {
 };
 
 /* Query flags. */
-#define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 leaf screen with a proven error margin,
-                                      FP64 recheck: identical results, fewer bytes
-                                      (the traversal's screen, ndim <= 16, for eps = 0
-                                      only; ignored for eps > 0).
+#define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 screen with a proven error margin, FP64
+                                      recheck: identical results (p = 2, eps = 0 only;
+                                      ignored for eps > 0).  ndim <= 16: batches of
+                                      8192 queries or more (rows of 1-4, 8 or 16
+                                      coordinates, k <= 32) take the tile walk (see
+                                      KDT_QUERY_TILE_WALK), others the traversal's
+                                      own leaf screen.
                                       For 32 <= ndim <= 64, p = 2, eps = 0, k <= 32:
                                       matrix-core screen over every point (rows
                                       rounded to fp16 in a per-tree frame, with a
@@ -81,6 +84,15 @@ enum kdt_error {
                                       matrix-core path: skip the traversal of the
                                       queries the certificate does not cover, whose
                                       output rows are then left as they were */
+#define KDT_QUERY_TILE_WALK 0x400u /* with KDT_QUERY_FP32_SCREEN, rows of 1-4, 8 or 16
+                                       coordinates, p = 2, eps = 0, k <= 32: the tile walk
+                                       (one tree walk per 64 queries, FP32 screen of the
+                                       points it reaches with a proven margin, FP64
+                                       recheck and visit certificate) for any batch size;
+                                       without this bit it is taken from 8192 queries on */
+#define KDT_QUERY_NO_TILE_WALK 0x800u /* with KDT_QUERY_FP32_SCREEN: never the tile walk (the
+                                          traversal's own FP32 leaf screen instead); both
+                                          give the same results */
 #define KDT_QUERY_NO_ORDER 0x200u /* keep the given query order: batches of 8192 or more
                                      queries otherwise run grouped by leaf (a leaf-order
                                      sort with stream-ordered scratch); results are the

@@ -665,6 +665,37 @@ __global__ void k_norms32(const double* __restrict__ rows, uint64_t n, uint32_t 
   atomicMax(pmax_bits, __float_as_uint(mx));  // non-negative floats order as integers
 }
 
+// The tile walk's rows (rows of up to 16 coordinates): row r in tree order as
+// rows32c[r] = fl32(fl64(x - c)) (c the centre of the tree's frame), its norm
+// pn32c[r] = fl32 of the FP64 sum of those values squared, and 8 padding rows
+// after the last (zero coordinates, NaN norm: a score read past a leaf's end
+// is NaN, which passes no test).  *pc_bits = max over rows of |x - c| (FP64),
+// rounded up, as f32 bits (the same bound k_rows_h forms for wider rows).
+__global__ void k_rows32c(const double* __restrict__ rows, uint64_t n, uint32_t D,
+                          const double* __restrict__ frame, float* __restrict__ rows32c,
+                          float* __restrict__ pn32c, uint32_t* pc_bits) {
+  float mx = 0.0f;
+  for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n + 8;
+       r += (uint64_t)gridDim.x * blockDim.x) {
+    double s = 0.0, pc = 0.0;
+    for (uint32_t c = 0; c < D; ++c) {
+      const double y = r < n ? rows[r * D + c] - frame[c] : 0.0;
+      const float f = (float)y;
+      rows32c[r * D + c] = f;
+      s += (double)f * (double)f;
+      pc += y * y;
+    }
+    pn32c[r] = r < n ? (float)s : __builtin_nanf("");
+    if (r < n) {
+      const double nr = sqrt(pc) * (1.0 + 0x1p-44);
+      float f = (float)nr;
+      if ((double)f < nr) f = nextafterf(f, INFINITY);
+      mx = fmaxf(mx, f);
+    }
+  }
+  atomicMax(pc_bits, __float_as_uint(mx));  // non-negative floats order as integers
+}
+
 // Leaf buckets: rows of each leaf stored coordinate-major, f64 and f32.
 __global__ __launch_bounds__(64) void k_leaf_buckets(const double* __restrict__ rows, uint32_t D,
                                                      const uint32_t* __restrict__ leaves /*start,len*/,
@@ -804,12 +835,14 @@ struct DevBuf {
 //   rows     f64[n][D]            tree order
 //   pts32    f32[n][D]            D <= 16: FP32 leaf buckets, same layout as pts
 //   pn32     f32[n]               D <= 16: fl32(|fl32(row)|^2), tree order
-//   frame    f64[D + 1]           8 <= D <= 64: matrix-core screen frame (centre, sigma)
+//   rows32c  f32[n + 8][D]        D <= 16: the tile walk's centred FP32 rows, tree order
+//   pn32c    f32[n + 8]           D <= 16: their norms (NaN on the 8 padding rows)
+//   frame    f64[D + 1]           D <= 64: the screens' frame (centre, sigma)
 //   rows_h   f16[n_pad][S]        8 <= D <= 64: matrix-core screen rows
 //   pnorm_h  u32x2[n_pad]         8 <= D <= 64: their norms as norm-block words
 // (offsets: image_layout below; one hipMalloc and one hipFree per tree).
 struct ImageLayout {
-  size_t nodes, boxes, pts, idx, rows, pts32, pn32, frame, rows_h, pnorm_h, total;
+  size_t nodes, boxes, pts, idx, rows, pts32, pn32, rows32c, pn32c, frame, rows_h, pnorm_h, total;
 };
 
 size_t image_piece(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~size_t(255); }
@@ -830,9 +863,11 @@ ImageLayout image_layout(uint64_t n, uint64_t D, uint64_t n_nodes, bool narrow, 
   if (narrow) {
     put(L.pts32, n * D * 4);
     put(L.pn32, n * 4);
+    put(L.rows32c, (n + 8) * D * 4);
+    put(L.pn32c, (n + 8) * 4);
   }
+  if (narrow || wide) put(L.frame, (D + 1) * 8);
   if (wide) {
-    put(L.frame, (D + 1) * 8);
     put(L.rows_h, n_pad * stride * 2);
     put(L.pnorm_h, n_pad * 8);
   }
@@ -917,7 +952,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<LvlNode> d_lvl;
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
-  DevBuf<float> pts32, pn32;
+  DevBuf<float> pts32, pn32, rows32c, pn32c;
   DevBuf<double> frame;
   DevBuf<uint2> pnorm_h;
   DevBuf<uint16_t> rows_h;
@@ -947,9 +982,11 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   if (narrow) {
     pts32.place(image, IL.pts32);
     pn32.place(image, IL.pn32);
+    rows32c.place(image, IL.rows32c);
+    pn32c.place(image, IL.pn32c);
   }
+  if (narrow || wide) frame.place(image, IL.frame);
   if (wide) {
-    frame.place(image, IL.frame);
     rows_h.place(image, IL.rows_h);
     pnorm_h.place(image, IL.pnorm_h);
   }
@@ -1042,14 +1079,16 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   k_order_by_box<<<(uint32_t)((t.n_nodes + 255) / 256), 256, 0, s>>>(nodes.p, t.n_nodes, boxes.p, Du,
                                                                     node_len.p);
   k_leaf_buckets<<<(uint32_t)(leaves.size() / 2), 64, 0, s>>>(rows.p, Du, d_leaves.p, pts.p, pts32.p);
+  if (narrow || wide) k_screen_frame<<<1, 64, 0, s>>>(boxes.p, Du, frame.p);
   if (wide) {
-    k_screen_frame<<<1, 64, 0, s>>>(boxes.p, Du, frame.p);
     k_rows_h<<<(uint32_t)std::min<uint64_t>((n_pad + 255) / 256, 8192), 256, 0, s>>>(
         rows.p, n, Du, stride32, n_pad, frame.p, rows_h.p, pnorm_h.p, reinterpret_cast<uint32_t*>(err.p) + 3);
   }
   if (narrow) {
     k_norms32<<<(uint32_t)std::min<uint64_t>((n + 255) / 256, 8192), 256, 0, s>>>(
         rows.p, n, Du, pn32.p, reinterpret_cast<uint32_t*>(err.p) + 2);
+    k_rows32c<<<(uint32_t)std::min<uint64_t>((n + 8 + 255) / 256, 8192), 256, 0, s>>>(
+        rows.p, n, Du, frame.p, rows32c.p, pn32c.p, reinterpret_cast<uint32_t*>(err.p) + 3);
   }
   k_validate_nodes<<<(uint32_t)std::min<uint64_t>((t.n_nodes + 255) / 256, 4096), 256, 0, s>>>(
       nodes.p, t.n_nodes, n, Du, err.p);
@@ -1083,6 +1122,8 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   t.pts = pts.release();
   t.pts32 = pts32.release();
   t.pn32 = pn32.release();
+  t.rows32c = rows32c.release();
+  t.pn32c = pn32c.release();
   t.idx = idx.release();
   t.rows = rows.release();
   t.frame = frame.release();
@@ -1093,12 +1134,12 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
     std::memcpy(&pm, &h_pmax[0], 4);
     t.pmax = (double)pm * (1.0 + 1e-6);
   }
-  if (wide) {
+  if (narrow || wide) {
     float pc;
     std::memcpy(&pc, &h_pmax[1], 4);
-    t.scr_pc = (double)pc;  // rounded up in k_rows_h
-    t.scr_sigma = h_sigma;
+    t.scr_pc = (double)pc;  // rounded up in k_rows_h / k_rows32c
   }
+  if (wide) t.scr_sigma = h_sigma;
   t.image = image;
   t.image_bytes = IL.total;
   image = nullptr;  // the tree owns it now

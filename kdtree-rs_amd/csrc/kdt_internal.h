@@ -9,11 +9,16 @@
 //   idx    u32[n_points]               original row of tree position
 //   rows   f64[n_points][ndim]         rows in tree order (row-major), the
 //                                      reference's Tree::data
-//   frame  f64[ndim + 1]               8 <= ndim <= 64 only: the matrix-core
-//                                      screen's frame, centre c (middle of the
-//                                      root box per coordinate), then the scale
-//                                      sigma = 2^s that maps the rows' largest
-//                                      |x - c| below 2^8
+//   rows32c f32[n + 8][ndim]           ndim <= 16 only: rows in tree order as
+//                                      fl32(fl64(x - c)), row-major, 8 zero
+//                                      padding rows (the tile walk, kdt_walk.hip)
+//   pn32c  f32[n + 8]                  ndim <= 16 only: their norms fl32(|row|^2)
+//                                      (NaN on the padding rows)
+//   frame  f64[ndim + 1]               ndim <= 64 only: the screens' frame,
+//                                      centre c (middle of the root box per
+//                                      coordinate), then the scale sigma = 2^s
+//                                      that maps the rows' largest |x - c|
+//                                      below 2^8 (the matrix-core screen's)
 //   rows_h f16[n_pad][stride]          8 <= ndim <= 64 only: rows in tree order as
 //                                      fp16(fl32(sigma (x - c))) (fp16 subnormals
 //                                      flushed to zero), zero-padded to stride 16,
@@ -67,7 +72,9 @@ struct Tree {
   uint16_t* rows_h = nullptr;  // matrix-core screen rows, fp16 bit patterns
   uint2* pnorm_h = nullptr;    // their norms, norm-block operand words
   double scr_sigma = 1.0;      // the frame's sigma (host copy)
-  double scr_pc = 0.0;         // upper bound on max over rows of |x - c| (FP64 rows)
+  double scr_pc = 0.0;         // upper bound on max over rows of |x - c| (FP64 rows), ndim <= 64
+  float* rows32c = nullptr;    // ndim <= 16: the tile walk's centred FP32 rows
+  float* pn32c = nullptr;      // ndim <= 16: their norms
   float* pn32 = nullptr;  // ndim <= 16: fl32(|row32|^2) per tree position, the FP32 leaf screen's norms
   double pmax = 0.0;     // ndim <= 16: upper bound on max |row32|
   void* image = nullptr;  // the one device allocation every array above lives in (kdt_build.hip)
@@ -177,7 +184,16 @@ struct QueryParams {
   bool allow_order = true;    // large batches may run in leaf order
   bool allow_scratch = true;  // scratch allowed, stream-ordered or from a region (the matrix-core
                               // screen needs it); false for a pure enqueue with no region
+  int walk = 0;               // tile walk for the FP32 screen (ndim <= 16): 0 = for batches of
+                              // WALK_MIN_QUERIES or more, 1 = any batch, -1 = never
 };
+
+// Batches from which the FP32 screen of rows of up to 16 coordinates takes the
+// tile walk (kdt_walk.hip) rather than the traversal's own leaf screen.
+constexpr uint64_t WALK_MIN_QUERIES = 8192;
+
+// Whether a batch runs in leaf order (kdt_query.hip): large enough and allowed.
+bool order_applies(const QueryParams& qp);
 
 // Lanes per query requested by KDT_QUERY_GROUP bits of the query flags (0 = none).
 // Lanes per query of the traversal: the narrowest group that holds the k list
@@ -267,6 +283,18 @@ bool screen_applies(const Tree& t, const QueryParams& qp);
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
                   std::string& msg);
 size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp);
+// The FP64 recheck with its visit certificate over per-query candidate lists
+// (cnt[qi] tree positions at cand[qi * 64], at most 64; flag[qi] = 1 for a
+// query the screen leaves to the traversal, *nflag of them), then the exact
+// traversal of every flagged query (kdt_screen.hip).
+int finish_candidates(const Tree& t, const QueryParams& qp, const KArgs& trav, const uint32_t* cnt,
+                      const uint32_t* cand, uint8_t* flag, uint32_t* nflag, hipStream_t s, std::string& msg);
+
+// Tile walk for rows of up to 16 coordinates (kdt_walk.hip).
+bool walk_applies(const Tree& t, const QueryParams& qp);
+int launch_walk(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
+                std::string& msg);
+size_t walk_scratch_bytes(const Tree& t, const QueryParams& qp);
 
 // Non-finite check of n doubles; sets *d_flag to 1 if any is NaN or infinite.
 void launch_check_finite(const double* d_x, uint64_t n, int* d_flag, hipStream_t s);

@@ -10,7 +10,8 @@
 // array's device address is looked up once with hipGetSymbolAddress and kept
 // per device; every kernel argument and every region handed to the query is
 // that address.  Each call carves its pieces (finiteness flag, leaf-order
-// sort, matrix-core screen lists) from the start of the array; a call whose
+// sort, matrix-core screen lists, tile-walk buffers) from the start of the
+// array (a tile walk that does not fit runs as the traversal's FP32 screen); a call whose
 // pieces need more than KDT_NOALLOC_SCRATCH_BYTES is rejected before any
 // launch.  Calls that overlap on one GPU (from several streams or threads,
 // before the previous call's work has finished) share the array and race:
@@ -64,7 +65,13 @@ extern "C" int kdt_query_device_noalloc(const kdt_tree* tree, const double* d_qu
   const bool check = (flags & KDT_QUERY_ASSUME_FINITE) == 0;
   if (!check && scalar_rc) return kdt::fail_api(scalar_rc, smsg);
   const kdt::Tree& t = tree->t;
-  const size_t need = (check ? kdt::scratch_piece(4) : 0) + kdt::query_scratch_bytes(t, qp);
+  size_t need = (check ? kdt::scratch_piece(4) : 0) + kdt::query_scratch_bytes(t, qp);
+  if (need > NOALLOC_BYTES && qp.walk == 0 && kdt::walk_applies(t, qp)) {
+    // The tile walk's buffers (per resident wavefront) do not fit: the
+    // traversal's own FP32 screen gives the same results with less scratch.
+    qp.walk = -1;
+    need = (check ? kdt::scratch_piece(4) : 0) + kdt::query_scratch_bytes(t, qp);
+  }
   if (need > NOALLOC_BYTES)
     return kdt::fail_api(KDT_ERR_ARGUMENT, "query needs " + std::to_string(need) +
                                                " bytes of scratch; kdt_query_device_noalloc holds " +

@@ -123,6 +123,7 @@ bool order_applies(const QueryParams& qp) {
 size_t query_scratch_bytes(const Tree& t, const QueryParams& qp) {
   if (qp.nq == 0) return 0;
   if (screen_applies(t, qp)) return screen_scratch_bytes(t, qp);
+  if (walk_applies(t, qp)) return walk_scratch_bytes(t, qp);
   if (order_applies(qp)) return scratch_piece(qp.nq * 4) + order_scratch_bytes(t, qp.nq);
   return 0;
 }
@@ -135,6 +136,7 @@ int launch_query(const Tree& t, const QueryParams& qp, const double* d_q, double
   Scratch local(s);  // stream-ordered scratch when no region is given
   Scratch& sc = region ? *region : local;
   if (screen_applies(t, qp)) return launch_screen(t, qp, a, sc, s, msg);
+  if (walk_applies(t, qp)) return launch_walk(t, qp, a, sc, s, msg);
 
   // Large batches run grouped by leaf (see kdt_order.hip).
   if (order_applies(qp)) {

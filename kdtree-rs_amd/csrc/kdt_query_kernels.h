@@ -924,7 +924,10 @@ inline void launch_reg(const KArgs& a, hipStream_t s) {
   constexpr int GPB = 256 / G;
   // One group per query (measured faster than a resident grid that hands
   // each group several queries); a larger batch loops (knn_body's next()).
-  const uint64_t blocks = std::min<uint64_t>((a.nq + GPB - 1) / GPB, 1u << 30);
+  uint64_t blocks = std::min<uint64_t>((a.nq + GPB - 1) / GPB, 1u << 30);
+  // Only the queries a screen left over (a.only): usually none or a few, so
+  // the groups of 16384 workgroups each look through a share of the flags.
+  if (a.only && blocks > (1u << 14)) blocks = 1u << 14;
   KernelTimer tm(s);
   if (a.stk <= 32) knn_reg_kernel<DT, G, M, SCREEN, 32><<<(uint32_t)blocks, 256, 0, s>>>(a);
   else knn_reg_kernel<DT, G, M, SCREEN, STK_MAX><<<(uint32_t)blocks, 256, 0, s>>>(a);

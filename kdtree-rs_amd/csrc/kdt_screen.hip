@@ -71,8 +71,6 @@
 
 namespace kdt {
 
-int launch_knn_lds(const KArgs& a, int metric, hipStream_t s);
-
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -1018,6 +1016,11 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     tm.stop("k_screen_select", (double)nq);
   }
 
+  return finish_candidates(t, qp, trav, cnt, cand, flag, nflag, s, msg);
+}
+
+int finish_candidates(const Tree& t, const QueryParams& qp, const KArgs& trav, const uint32_t* cnt,
+                      const uint32_t* cand, uint8_t* flag, uint32_t* nflag, hipStream_t s, std::string& msg) {
   CheckArgs ca{};
   ca.q = trav.q;
   ca.rows = t.rows;
@@ -1030,29 +1033,39 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   ca.nflag = nflag;
   ca.out_d = trav.out_d;
   ca.out_i = trav.out_i;
-  ca.nq = nq;
+  ca.nq = qp.nq;
   ca.D = (uint32_t)t.d;
-  ca.k = k;
+  ca.k = (uint32_t)qp.k;
   ca.n = (uint32_t)t.n;
   ca.leafsize = (uint32_t)std::min<uint64_t>(t.leafsize, 0xFFFFFFFFu);
   ca.cutoff = qp.cutoff;
   {
     KernelTimer tm(s);
-    k_screen_recheck<<<(uint32_t)nq, 64, (size_t)k * t.d * 8, s>>>(ca);
-    tm.stop("k_screen_recheck", (double)nq);
+    k_screen_recheck<<<(uint32_t)qp.nq, 64, (size_t)qp.k * t.d * 8, s>>>(ca);
+    tm.stop("k_screen_recheck", (double)qp.nq);
   }
 
-  // Flagged queries: the exact traversal.
+  // Flagged queries: the exact traversal (the compiled register-list kernel
+  // for its widths, the runtime-width one otherwise), in the given order.
   KArgs fb = trav;
   fb.order = nullptr;
   fb.only = flag;
   fb.only_count = nflag;
-  const int rc = qp.screen_only ? 0 : launch_knn_lds(fb, M_L2, s);
+  int rc = 0;
+  if (!qp.screen_only) {
+    rc = -1;
+    if (qp.k <= 64) {
+      int G = group_width(qp.k, qp.group);
+      if (t.d >= 32 && G < 32) G = 32;
+      rc = launch_knn_l2(fb, G, false, s);
+    }
+    if (rc) rc = launch_knn_lds(fb, M_L2, s);
+  }
   if (rc) {
     msg = "no fallback kernel";
     return 101;
   }
-  e = hipGetLastError();
+  const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     msg = std::string("screen launch: ") + hipGetErrorString(e);
     return 100;

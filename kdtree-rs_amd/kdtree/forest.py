@@ -32,6 +32,8 @@ KDT_QUERY_ASSUME_FINITE = 0x2
 KDT_QUERY_MATRIX_SCREEN = 0x8
 KDT_QUERY_GROUP_SHIFT = 4
 KDT_QUERY_NO_ORDER = 0x200
+KDT_QUERY_TILE_WALK = 0x400
+KDT_QUERY_NO_TILE_WALK = 0x800
 KDT_BATCH_PER_TREE_LAUNCHES = 0x100
 KDT_BATCH_NO_ORDER = 0x200
 

@@ -696,6 +696,23 @@ __global__ void k_rows32c(const double* __restrict__ rows, uint64_t n, uint32_t 
   atomicMax(pc_bits, __float_as_uint(mx));  // non-negative floats order as integers
 }
 
+// The tile walk's node boxes (rows of up to 16 coordinates): each box in the
+// frame as FP32, rounded outwards -- lo32 <= lo - c and hi32 >= hi - c
+// exactly (the FP64 difference is rounded towards the outside in FP32, then
+// moved one more FP32 step outwards, which covers the FP64 difference's own
+// rounding) -- so that a box distance formed from them never exceeds the
+// exact one by more than the rounding of the query and of the arithmetic.
+__global__ void k_boxes32c(const double* __restrict__ boxes, uint64_t n_nodes, uint32_t D,
+                           const double* __restrict__ frame, float* __restrict__ boxes32c) {
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_nodes * 2 * D;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t j = (uint32_t)(e % D);
+    const bool hi = (e / D) % 2 == 1;
+    const double y = boxes[e] - frame[j];
+    boxes32c[e] = hi ? nextafterf(__double2float_ru(y), INFINITY) : nextafterf(__double2float_rd(y), -INFINITY);
+  }
+}
+
 // Leaf buckets: rows of each leaf stored coordinate-major, f64 and f32.
 __global__ __launch_bounds__(64) void k_leaf_buckets(const double* __restrict__ rows, uint32_t D,
                                                      const uint32_t* __restrict__ leaves /*start,len*/,
@@ -837,12 +854,13 @@ struct DevBuf {
 //   pn32     f32[n]               D <= 16: fl32(|fl32(row)|^2), tree order
 //   rows32c  f32[n + 8][D]        D <= 16: the tile walk's centred FP32 rows, tree order
 //   pn32c    f32[n + 8]           D <= 16: their norms (NaN on the 8 padding rows)
+//   boxes32c f32[n_nodes][2][D]   D <= 16: the tile walk's boxes (centred, rounded outwards)
 //   frame    f64[D + 1]           D <= 64: the screens' frame (centre, sigma)
 //   rows_h   f16[n_pad][S]        8 <= D <= 64: matrix-core screen rows
 //   pnorm_h  u32x2[n_pad]         8 <= D <= 64: their norms as norm-block words
 // (offsets: image_layout below; one hipMalloc and one hipFree per tree).
 struct ImageLayout {
-  size_t nodes, boxes, pts, idx, rows, pts32, pn32, rows32c, pn32c, frame, rows_h, pnorm_h, total;
+  size_t nodes, boxes, pts, idx, rows, pts32, pn32, rows32c, pn32c, boxes32c, frame, rows_h, pnorm_h, total;
 };
 
 size_t image_piece(size_t bytes) { return (std::max<size_t>(bytes, 1) + 255) & ~size_t(255); }
@@ -865,6 +883,7 @@ ImageLayout image_layout(uint64_t n, uint64_t D, uint64_t n_nodes, bool narrow, 
     put(L.pn32, n * 4);
     put(L.rows32c, (n + 8) * D * 4);
     put(L.pn32c, (n + 8) * 4);
+    put(L.boxes32c, n_nodes * 2 * D * 4);
   }
   if (narrow || wide) put(L.frame, (D + 1) * 8);
   if (wide) {
@@ -952,7 +971,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   DevBuf<LvlNode> d_lvl;
   DevBuf<Chunk> d_chunks;
   DevBuf<double> rows, trows, part, boxes, pts;
-  DevBuf<float> pts32, pn32, rows32c, pn32c;
+  DevBuf<float> pts32, pn32, rows32c, pn32c, boxes32c;
   DevBuf<double> frame;
   DevBuf<uint2> pnorm_h;
   DevBuf<uint16_t> rows_h;
@@ -984,6 +1003,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
     pn32.place(image, IL.pn32);
     rows32c.place(image, IL.rows32c);
     pn32c.place(image, IL.pn32c);
+    boxes32c.place(image, IL.boxes32c);
   }
   if (narrow || wide) frame.place(image, IL.frame);
   if (wide) {
@@ -1089,6 +1109,8 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
         rows.p, n, Du, pn32.p, reinterpret_cast<uint32_t*>(err.p) + 2);
     k_rows32c<<<(uint32_t)std::min<uint64_t>((n + 8 + 255) / 256, 8192), 256, 0, s>>>(
         rows.p, n, Du, frame.p, rows32c.p, pn32c.p, reinterpret_cast<uint32_t*>(err.p) + 3);
+    k_boxes32c<<<(uint32_t)std::min<uint64_t>((t.n_nodes * 2 * D + 255) / 256, 8192), 256, 0, s>>>(
+        boxes.p, t.n_nodes, Du, frame.p, boxes32c.p);
   }
   k_validate_nodes<<<(uint32_t)std::min<uint64_t>((t.n_nodes + 255) / 256, 4096), 256, 0, s>>>(
       nodes.p, t.n_nodes, n, Du, err.p);
@@ -1124,6 +1146,7 @@ int build_tree(Tree& t, const double* d_data, hipStream_t s, std::string& msg) {
   t.pn32 = pn32.release();
   t.rows32c = rows32c.release();
   t.pn32c = pn32c.release();
+  t.boxes32c = boxes32c.release();
   t.idx = idx.release();
   t.rows = rows.release();
   t.frame = frame.release();

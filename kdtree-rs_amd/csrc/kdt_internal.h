@@ -14,6 +14,8 @@
 //                                      padding rows (the tile walk, kdt_walk.hip)
 //   pn32c  f32[n + 8]                  ndim <= 16 only: their norms fl32(|row|^2)
 //                                      (NaN on the padding rows)
+//   boxes32c f32[n_nodes][2][ndim]     ndim <= 16 only: the boxes in the frame as
+//                                      FP32 rounded outwards (the tile walk)
 //   frame  f64[ndim + 1]               ndim <= 64 only: the screens' frame,
 //                                      centre c (middle of the root box per
 //                                      coordinate), then the scale sigma = 2^s
@@ -75,6 +77,7 @@ struct Tree {
   double scr_pc = 0.0;         // upper bound on max over rows of |x - c| (FP64 rows), ndim <= 64
   float* rows32c = nullptr;    // ndim <= 16: the tile walk's centred FP32 rows
   float* pn32c = nullptr;      // ndim <= 16: their norms
+  float* boxes32c = nullptr;   // ndim <= 16: node boxes in the frame, FP32, rounded outwards
   float* pn32 = nullptr;  // ndim <= 16: fl32(|row32|^2) per tree position, the FP32 leaf screen's norms
   double pmax = 0.0;     // ndim <= 16: upper bound on max |row32|
   void* image = nullptr;  // the one device allocation every array above lives in (kdt_build.hip)

@@ -32,16 +32,38 @@
 // d / s <= |q - p|^2 <= s d, s = 1 + 2 (D + 3) 2^-53.  Hence
 //   d <= U(A) = s (sqrt(A + E2) + e1)^2            (every point), and
 //   d <= b  =>  A <= T(b) = (sqrt(s b) + e1)^2 + E2.
-// (e1 and E2 are rounded up to FP32, U and T formed in FP64 and widened by
-// 2^-50, T rounded up to FP32: screen_t32.)
+// (e1 and E2 are rounded up to FP32; T(cutoff) is formed in FP64 and widened
+// by 2^-50, then rounded up to FP32: screen_t32.)  After an insert the lane
+// needs T(U(A_k)) and TB(U(A_k)) (box test, below) only as upper bounds, and
+// forms them in FP32 from one square root:
+//   x = fl(A + E2) (1 + 2^-22),  r = sqrt(x) (1 + 2^-21),
+//   t = ((r + e1) + e1) (1 + 2^-21),
+//   T' = (t^2 + E2 + 2^-147)(1 + 2^-21),
+//   TB' = (t^2 (1 + (D + 3) u) + (D + 1) 2^-149 + 2^-147)(1 + 2^-21),
+// every product and sum rounded to nearest: each factor 1 + 2^-21 (eight
+// FP32 roundings) covers the roundings of its stage and the sqrt's (at most
+// four units in the last place: v_sqrt_f32 is within one), and sqrt(s U) + e1
+// = s (sqrt(A + E2) + e1) + e1 <= t (s < 1 + 2^-47); the 2^-147 covers the
+// absolute rounding (2^-150 per operation) of a subnormal t^2.  So T' >= T(U(A_k)) and TB' >= TB(U(A_k)): a larger
+// threshold only appends or visits more.
+//
+// Box test.  The build stores every node box in the frame as FP32 rounded
+// outwards (lo32 <= lo - c, hi32 >= hi - c; boxes32c), and the lane forms
+//   bd32 = sum_j fl(g_j^2),  g_j = max(fl(lo32_j - q~_j), fl(q~_j - hi32_j), 0)
+// (fl(lo32_j - q~_j) = fma(0.5, m_j, lo32_j): m = -2 q~ exactly).  With
+// delta = q~ - (q - c) (|delta| <= e1 as for the rows) and gap the exact gap
+// vector of q to the box, g_j <= (1 + u)(gap_j + |delta_j|), so
+//   bd32 <= (1 + (D + 3) u) (sqrt(bd) + e1)^2 + (D + 1) 2^-149,
+// bd the exact box distance; a point p inside has bd <= |q - p|^2 <= s d.  A
+// node holding a point with d <= b therefore has bd32 <= TB(b) =
+// (1 + (D + 3) u)(sqrt(s b) + e1)^2 + (D + 1) 2^-149 (formed like T, from
+// the same t after an insert, rounded up to FP32).
 //
 // The lane keeps the K smallest A it has seen (la, K >= k), its bound
 // b = min(U(la[k-1]), cutoff) -- at least k points have d <= b, so b is never
 // below the k-th reference distance d_k -- and appends every reached point
-// with A <= T(b) to its buffer.  The wave skips a node when no lane has its
-// FP64 box distance <= b (1 + 2^-30) (the computed box distance is at most
-// (1 + (D + 2) 2^-53) above the exact one, which is at most |q - p|^2 <= s d
-// for every point p inside).  b only decreases.  So every point of the
+// with A <= T(b) to its buffer.  The wave skips a node when no lane has
+// bd32 <= TB(b).  b only decreases.  So every point of the
 // reference's top-k (d <= d_k <= b at all times) lies in a node the walk
 // enters, is appended, and has A <= T(b_final): the candidates -- the appended
 // points with A <= T(b_final) -- contain the top-k, and the recheck's FP64
@@ -68,6 +90,16 @@ constexpr uint32_t WALK_KMAX = 32;   // largest k on this path
 constexpr int WALK_WAVES = 4;        // independent wavefronts per workgroup
 constexpr uint32_t WALK_LP = 64;     // leaf points staged in LDS at a time
 
+// Waves per SIMD the walk is compiled for.  Rows of up to 8 coordinates: 4
+// (k <= 32: 128 VGPRs; the few values that do not fit are the list's starting
+// entries, reloaded once per tile, outside the walk's loops); rows of 16: as
+// many as the registers allow.
+#ifndef KDT_WALK_MIN_WAVES
+#define KDT_WALK_MIN_WAVES 4
+#endif
+template <int D>
+constexpr int walk_min_waves() { return D <= 8 ? KDT_WALK_MIN_WAVES : 1; }
+
 struct WParams {
   uint64_t nq, n_tiles, n_waves;
   uint32_t k;
@@ -88,40 +120,62 @@ __device__ __forceinline__ void list_insert(float (&la)[K], float A) {
   la[0] = fminf(la[0], A);
 }
 
-// One wavefront per tile of 64 consecutive queries of the processing order;
-// the grid's waves take contiguous ranges of tiles (XCD-grouped blocks, so
-// that an XCD's L2 holds one region of the tree).  Pointers are kernel
+// One wavefront per tile of 64 consecutive queries of the processing order,
+// a persistent grid of waves drawing tiles from per-XCD counters.  Pointers are kernel
 // arguments (not a struct) so that the compiler knows they do not alias and
 // reads the tree through wave-uniform scalar loads.
 template <int D, int K>
-__global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(
-    const DevNode* __restrict__ nodes, const double* __restrict__ boxes, const float* __restrict__ rows32c,
+__global__ __launch_bounds__(64 * WALK_WAVES, walk_min_waves<D>()) void k_walk(
+    const DevNode* __restrict__ nodes, const float* __restrict__ boxes32c, const float* __restrict__ rows32c,
     const float* __restrict__ pn32c, const double* __restrict__ frame, const double* __restrict__ q,
     const uint32_t* __restrict__ order, uint2* __restrict__ buf, uint32_t* __restrict__ cnt_out,
-    uint32_t* __restrict__ cand, uint8_t* __restrict__ flag, uint32_t* __restrict__ nflag, WParams p) {
+    uint32_t* __restrict__ cand, uint8_t* __restrict__ flag, uint32_t* __restrict__ nflag,
+    uint32_t* __restrict__ tile_ctr, WParams p) {
   __shared__ double s_rep[WALK_WAVES][D];  // the tile's representative query (near-child choice)
   __shared__ float s_pts[WALK_WAVES][WALK_LP * D];  // a staged piece of a leaf: rows, then
   __shared__ float s_pn[WALK_WAVES][WALK_LP];       // their norms
   const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint32_t full = gridDim.x & ~7u;
-  const uint32_t blk = blockIdx.x < full ? (blockIdx.x & 7u) * (full >> 3) + (blockIdx.x >> 3) : blockIdx.x;
-  const uint64_t wave = (uint64_t)blk * WALK_WAVES + w;
+  const uint64_t wave = (uint64_t)blockIdx.x * WALK_WAVES + w;
   if (wave >= p.n_waves) return;  // wave-uniform; the kernel has no barrier
   uint2* lb = buf + wave * WALK_CB * 64 + lane;  // entry e of this lane at lb[e * 64]
-  const uint64_t t0 = p.n_tiles * wave / p.n_waves, t1 = p.n_tiles * (wave + 1) / p.n_waves;
   const double shrink = p.shrink;
 
-  for (uint64_t tile = t0; tile < t1; ++tile) {
+  // Tiles are taken from one counter per XCD: XCD x (workgroups go to the 8
+  // XCDs in turn, workgroup b to XCD b mod 8) owns the x-th eighth of the
+  // tiles, so that its L2 holds one region of the tree, and its waves draw
+  // them one at a time (tiles in dense regions cost more); a wave whose XCD's
+  // eighth is done helps the others, so every tile is taken whatever the grid.
+  const uint32_t xcd = blockIdx.x & 7u;
+  uint32_t xi = 0;
+  auto next_tile = [&](uint64_t& tile) -> bool {
+    while (xi < 8) {
+      const uint32_t x = (xcd + xi) & 7u;
+      const uint64_t lo = p.n_tiles * x / 8, hi = p.n_tiles * (x + 1) / 8;
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(tile_ctr + x, 1u);
+      t = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+      if (lo + t < hi) {
+        tile = lo + t;
+        return true;
+      }
+      ++xi;
+    }
+    return false;
+  };
+
+  uint64_t tile = 0;
+  while (next_tile(tile)) {
     const uint64_t slot = tile * 64 + lane;
     const bool live = slot < p.nq;
     const uint64_t qi = live ? (order ? (uint64_t)order[slot] : slot) : 0;
-    double qd[D];
     float m[D];
     double qn64 = 0.0, qc2 = 0.0;
+    const uint32_t rep = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(live)) / 2;  // the middle live lane
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      qd[j] = q[qi * D + j];
-      const double y = qd[j] - frame[j];
+      const double qj = q[qi * D + j];
+      if (lane == rep) s_rep[w][j] = qj;
+      const double y = qj - frame[j];
       const float f = (float)y;
       m[j] = -2.0f * f;
       qn64 += (double)f * (double)f;
@@ -140,28 +194,35 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(
     float la[K];
 #pragma unroll
     for (int e = 0; e < K; ++e) la[e] = (uint32_t)e + p.k < (uint32_t)K ? -INFINITY : INFINITY;
-    double b = ok ? p.cutoff : -INFINITY;
-    float T = ok ? screen_t32(b, shrink, e1f, e2f) : -INFINITY;
-    uint32_t nb = 0;
-
-    // The representative: the middle live lane's query.
-    const uint32_t nlive = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(live));
-    if (lane == nlive / 2) {
-#pragma unroll
-      for (int j = 0; j < D; ++j) s_rep[w][j] = qd[j];
-    }
-    __builtin_amdgcn_wave_barrier();
-
-    // The lane's bound from its list (after an insert).
-    auto update = [&]() {
-      const float ak = la[K - 1];  // the k-th smallest A so far
-      double bb = p.cutoff;
-      if (ak < INFINITY) {
-        const double r = sqrt(fmax((double)ak, 0.0) + (double)e2f) + (double)e1f;
-        bb = fmin(bb, shrink * r * r * (1.0 + 0x1p-50));
+    const double tbf = 1.0 + (D + 3.0) * 0x1p-24;   // TB's rounding factor
+    const double tbt = (D + 1.0) * 0x1p-149;        // and its subnormal term
+    constexpr float tbf32 = 1.0f + (D + 3.0f) * 0x1p-24f, tbt32 = (D + 1.0f) * 0x1p-149f;  // exact in FP32
+    float T = -INFINITY, TB = -INFINITY;
+    if (ok) {  // the cutoff's thresholds (FP64, rounded up)
+      T = screen_t32(p.cutoff, shrink, e1f, e2f);
+      if (p.cutoff < INFINITY) {
+        const double t = sqrt(p.cutoff * shrink) + (double)e1f;
+        TB = __double2float_ru((tbf * t * t + tbt) * (1.0 + 0x1p-50));
+      } else {
+        TB = INFINITY;
       }
-      b = bb;
-      T = screen_t32(b, shrink, e1f, e2f);
+    }
+    uint32_t nb = 0;
+    __builtin_amdgcn_wave_barrier();  // s_rep written above
+
+    // The lane's thresholds from its list (after an insert): FP32, one square
+    // root (header).  They only decrease: T and TB of the cutoff bound them.
+    auto update = [&]() {
+      const float ak = la[K - 1];  // the k-th smallest A so far (+inf until k values)
+      if (ak < INFINITY) {
+        constexpr float up1 = 1.0f + 0x1p-22f, up2 = 1.0f + 0x1p-21f;
+        const float x = (fmaxf(ak, 0.0f) + e2f) * up1;
+        const float r = __builtin_amdgcn_sqrtf(x) * up2;
+        const float t = ((r + e1f) + e1f) * up2;
+        const float tt2 = t * t;
+        T = fminf(T, (tt2 + e2f + 0x1p-147f) * up2);
+        TB = fminf(TB, (tt2 * tbf32 + tbt32 + 0x1p-147f) * up2);
+      }
     };
     // A reached point: append under the threshold, insert into the list.
     auto offer = [&](float A, uint32_t pos) {
@@ -191,8 +252,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(
             nb = keep;
             if (nb > WALK_CB / 2) {  // still crowded: leave the query to the traversal
               ok = false;
-              b = -INFINITY;
               T = -INFINITY;
+              TB = -INFINITY;
             }
           }
         }
@@ -201,17 +262,19 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(
 
     // The walk is wave-uniform; readfirstlane states it, so that the node,
     // box and row reads are scalar loads.
+    // The walk is wave-uniform; readfirstlane states it, so that the node,
+    // box and row reads are scalar loads.
     uint32_t node = 0, sp = 0, stk = 0;  // stack entry i in lane i's stk (depth <= 32)
     for (;;) {
       node = (uint32_t)__builtin_amdgcn_readfirstlane((int)node);
-      const double* bx = boxes + (size_t)node * 2 * D;
-      double bd = 0.0;
+      const float* bx = boxes32c + (size_t)node * 2 * D;
+      float bd = 0.0f;
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        const double g = fmax(fmax(bx[j] - qd[j], qd[j] - bx[D + j]), 0.0);
-        bd += g * g;
+        const float g = fmaxf(fmaxf(__builtin_fmaf(0.5f, m[j], bx[j]), -__builtin_fmaf(0.5f, m[j], bx[D + j])), 0.0f);
+        bd = __builtin_fmaf(g, g, bd);
       }
-      if (__builtin_amdgcn_ballot_w64(bd <= b * (1.0 + 0x1p-30))) {
+      if (__builtin_amdgcn_ballot_w64(bd <= TB)) {
         const DevNode nd = nodes[node];
         if (!(nd.b & LEAF_BIT)) {
           const bool left = __builtin_amdgcn_readfirstlane(s_rep[w][nd.b & DIM_MASK] <= nd.split ? 1 : 0) != 0;
@@ -332,6 +395,9 @@ uint64_t walk_waves(uint64_t d, uint32_t k, uint64_t n_tiles) {
     per_dev = c;
   }
   if (per_dev <= 0) per_dev = 1024;
+#ifdef KDT_WALK_WAVES_PER_CU
+  per_dev = std::min(per_dev, 256 * KDT_WALK_WAVES_PER_CU);
+#endif
   return std::max<uint64_t>(1, std::min<uint64_t>(n_tiles, (uint64_t)per_dev));
 }
 
@@ -341,7 +407,7 @@ bool walk_width(uint64_t d) { return d == 1 || d == 2 || d == 3 || d == 4 || d =
 
 bool walk_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.allow_scratch && qp.metric == M_L2 && qp.eps_factor == 1.0 &&
-         qp.k <= (uint64_t)WALK_KMAX && t.rows32c && t.pn32c && t.frame && walk_width(t.d) &&
+         qp.k <= (uint64_t)WALK_KMAX && t.rows32c && t.pn32c && t.boxes32c && t.frame && walk_width(t.d) &&
          t.scr_pc <= 1e18 && t.n < (1ull << 32) && qp.nq < (1ull << 40) &&
          (qp.walk > 0 || (qp.walk == 0 && qp.nq >= WALK_MIN_QUERIES));
 }
@@ -349,8 +415,8 @@ bool walk_applies(const Tree& t, const QueryParams& qp) {
 size_t walk_scratch_bytes(const Tree& t, const QueryParams& qp) {
   const uint64_t nq = qp.nq;
   const uint64_t waves = walk_waves(t.d, (uint32_t)qp.k, (nq + 63) / 64);
-  size_t b = scratch_piece(waves * WALK_CB * 64 * 8) + scratch_piece(nq * 4) + scratch_piece(nq * WALK_CAND * 4) +
-             scratch_piece(nq) + scratch_piece(4);
+  size_t b = scratch_piece(waves * WALK_CB * 64 * 8) + scratch_piece(9 * 4) + scratch_piece(nq * 4) +
+             scratch_piece(nq * WALK_CAND * 4) + scratch_piece(nq);
   if (order_applies(qp)) b += scratch_piece(nq * 4) + order_scratch_bytes(t, nq);
   return b;
 }
@@ -375,15 +441,15 @@ int launch_walk(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch
   uint32_t *cnt = nullptr, *cand = nullptr, *nflag = nullptr;
   uint8_t* flag = nullptr;
   hipError_t e = sc.get(&buf, waves * WALK_CB * 64);
+  if (e == hipSuccess) e = sc.get(&nflag, 9);  // the flagged count, then the 8 tile counters
   if (e == hipSuccess) e = sc.get(&cnt, nq);
   if (e == hipSuccess) e = sc.get(&cand, nq * WALK_CAND);
   if (e == hipSuccess) e = sc.get(&flag, nq);
-  if (e == hipSuccess) e = sc.get(&nflag, 1);
   if (e != hipSuccess) {
     msg = std::string("walk scratch: ") + hipGetErrorString(e);
     return 100;
   }
-  launch_zero_u32(nflag, 1, s);
+  launch_zero_u32(nflag, 9, s);
   WParams wp{};
   wp.nq = nq;
   wp.n_tiles = n_tiles;
@@ -396,8 +462,8 @@ int launch_walk(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch
   {
     KernelTimer tm(s);
 #define KDT_WALK_LAUNCH(DD, KK)                                                                              \
-  k_walk<DD, KK><<<blocks, 64 * WALK_WAVES, 0, s>>>(t.nodes, t.boxes, t.rows32c, t.pn32c, t.frame, trav.q, \
-                                                    order, buf, cnt, cand, flag, nflag, wp)
+  k_walk<DD, KK><<<blocks, 64 * WALK_WAVES, 0, s>>>(t.nodes, t.boxes32c, t.rows32c, t.pn32c, t.frame, trav.q, \
+                                                    order, buf, cnt, cand, flag, nflag, nflag + 1, wp)
 #define KDT_WALK_D(DD)                                  \
   case DD:                                              \
     if (k <= 8) KDT_WALK_LAUNCH(DD, 8);                 \

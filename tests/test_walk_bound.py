@@ -18,15 +18,25 @@ even, subnormals), and the checks are the facts the walk relies on:
   inclusion  A <= T(d), d the reference's FP64 reduced distance (the oracle's
              point_dist): a point with d <= b is appended and is a candidate
   upper      d <= U(A): k points seen bound the k-th distance by U(A_k)
-  box        the FP64 box distance of any box holding p is <= d (1 + 2^-30):
-             a node holding a point with d <= b is never skipped
+  box        bd32 <= TB(d) for any box holding p, bd32 the walk's FP32 box
+             distance over the build's outward-rounded centred FP32 box
+             (k_boxes32c) and TB(b) = fl32_up(((1 + (D + 3) u)(sqrt(s b) + e1)^2
+             + (D + 1) 2^-149)(1 + 2^-50)): a node holding a point with d <= b
+             is never skipped
+  update     after an insert the kernel forms T and TB in FP32 from one square
+             root (kdt_walk.hip header), with the square root taken one unit
+             in the last place low here (v_sqrt_f32 is within one): they are
+             never below T(U(A)) and TB(U(A))
 Inputs stress each: near-equal rows far from the centre (the row rounding is
 the whole error), cancellation at large norms, subnormal-range values, the
 1e18 limit, offset data.  Margins with a term taken out must break on them:
 E2 without its N^2 term or with u N^2 in place of (D + 3) u N^2 (core), U
-without E2 (upper), T without E2 (inclusion).  (The row-rounding term e1 is
+without E2 (upper), T without E2 (inclusion), TB without its rounding terms
+(TB = s d: box).  (The row-rounding term e1 is
 needed by the proof but stays below E2's N^2 term -- 2 e1 sqrt(d) <= 2 u N^2 --
-so no input breaks a margin that drops e1 alone.)
+so no input breaks a margin that drops e1 alone; likewise TB's e1 and (D + 3)
+u terms are each covered by the outward rounding of the boxes or by the
+other on these inputs, so the negative test drops both.)
 """
 
 from __future__ import annotations
@@ -128,12 +138,49 @@ def U_of(D, A, e1, e2):
     return shrink(D) * r * r * (1.0 + 2.0 ** -50)
 
 
-def box_dist(q, lo, hi):
-    s = 0.0
-    for j in range(q.size):
-        g = max(max(lo[j] - q[j], q[j] - hi[j]), 0.0)
-        s += g * g
-    return s
+def f32_down(x: float) -> float:
+    f = np.float32(x)
+    if float(f) > x:
+        f = np.nextafter(f, np.float32(-np.inf))
+    return float(f)
+
+
+def box32(lo, hi, c):
+    """k_boxes32c: fl64(lo - c) rounded down to FP32 and one step further, hi up."""
+    lo32 = [float(np.nextafter(np.float32(f32_down(float(v))), np.float32(-np.inf))) for v in lo - c]
+    hi32 = [float(np.nextafter(np.float32(f32_up(float(v))), np.float32(np.inf))) for v in hi - c]
+    return lo32, hi32
+
+
+def walk_box_dist(q32, lo32, hi32) -> Fraction:
+    """The kernel's FP32 box distance: g_j = max(fma(0.5, m_j, lo32_j),
+    -fma(0.5, m_j, hi32_j), 0) with m = -2 q~, then fma(g, g, acc)."""
+    acc = Fraction(0)
+    for x, lo, hi in zip(q32, lo32, hi32):
+        m = Fraction(float(np.float32(-2.0) * x))
+        g = max(round_f32(m / 2 + Fraction(lo)), -round_f32(m / 2 + Fraction(hi)), Fraction(0))
+        acc = round_f32(g * g + acc)
+    return acc
+
+
+def TB_of(D, b, e1, factor=True):
+    t = math.sqrt(b * shrink(D)) + e1
+    f = 1.0 + (D + 3.0) * 2.0 ** -24 if factor else 1.0
+    return f32_up((f * t * t + (D + 1.0) * 2.0 ** -149) * (1.0 + 2.0 ** -50))
+
+
+def after_insert(D, A, e1, e2):
+    """T and TB as the kernel forms them after an insert: FP32 operations
+    (numpy float32, round to nearest), the square root one ulp low."""
+    f = np.float32
+    up1, up2 = f(1.0 + 2.0 ** -22), f(1.0 + 2.0 ** -21)
+    x = (f(max(float(A), 0.0)) + f(e2)) * up1
+    r = np.nextafter(np.sqrt(x), f(0.0)) * up2
+    t = ((r + f(e1)) + f(e1)) * up2
+    tt2 = t * t
+    T = (tt2 + f(e2) + f(2.0 ** -147)) * up2
+    TB = (tt2 * f(1.0 + (D + 3.0) * 2.0 ** -24) + f((D + 1.0) * 2.0 ** -149) + f(2.0 ** -147)) * up2
+    return float(T), float(TB)
 
 
 def cases():
@@ -163,7 +210,7 @@ def cases():
     return out
 
 
-def evaluate(D, q, p, extra, drop=None):
+def evaluate(D, q, p, extra, drop=None, with_frame=False):
     pts = np.vstack([p[None, :], extra])
     c, pc = frame(np.vstack([pts, q[None, :]]))  # the query inside the root box, as in most calls
     q32, p32 = centred(q, c), centred(p, c)
@@ -171,23 +218,69 @@ def evaluate(D, q, p, extra, drop=None):
     exact = sum((Fraction(float(x)) - Fraction(float(y))) ** 2 for x, y in zip(q32, p32))
     E2, e1f, e2f = margins(D, q, c, pc, drop)
     d = point_dist(q, p)
+    if with_frame:
+        return A, exact, E2, e1f, e2f, d, c, q32
     return A, exact, E2, e1f, e2f, d
+
+
+def boxes_around(rng, p, q, D):
+    """Boxes holding p: from p and a corner nearby, and reaching towards q."""
+    out = []
+    for _ in range(2):
+        r = p + rng.standard_normal(D) * (1 + np.abs(p)) * 1e-3
+        out.append((np.minimum(p, r), np.maximum(p, r)))
+    r = p + 0.999 * (q - p)
+    out.append((np.minimum(p, r), np.maximum(p, r)))
+    # p the box's corner nearest to q: the exact box distance is |q - p|^2
+    ext = np.abs(rng.standard_normal(D)) * (1 + np.abs(p))
+    away = np.where(q < p, p + ext, p - ext)
+    out.append((np.minimum(p, away), np.maximum(p, away)))
+    return out
 
 
 def test_walk_margin_holds():
     rng = np.random.default_rng(11)
     for D, q, p, extra in cases():
-        A, exact, E2, e1f, e2f, d = evaluate(D, q, p, extra)
+        A, exact, E2, e1f, e2f, d, c, q32 = evaluate(D, q, p, extra, with_frame=True)
         assert abs(A - exact) <= Fraction(E2), (D, float(A), float(exact), E2)
         T = T_of(D, d, e1f, e2f)
         assert A <= Fraction(T), (D, float(A), d, T)
         U = U_of(D, A, e1f, e2f)
         assert d <= U, (D, d, U, float(A))
-        # boxes holding p, from p and a random corner
-        for _ in range(3):
-            r = p + rng.standard_normal(D) * (1 + np.abs(p)) * 1e-3
-            lo, hi = np.minimum(p, r), np.maximum(p, r)
-            assert box_dist(q, lo, hi) <= d * (1.0 + 2.0 ** -30), (D, box_dist(q, lo, hi), d)
+        Ti, TBi = after_insert(D, A, e1f, e2f)
+        assert Ti >= T_of(D, U, e1f, e2f) and TBi >= TB_of(D, U, e1f), (D, float(A))
+        for lo, hi in boxes_around(rng, p, q, D):
+            lo32, hi32 = box32(lo, hi, c)
+            bd32 = walk_box_dist(q32, lo32, hi32)
+            assert bd32 <= Fraction(TB_of(D, d, e1f)), (D, float(bd32), d)
+
+
+def test_weakened_box_threshold_fails():
+    rng = np.random.default_rng(12)
+    broken = 0
+    for D, q, p, extra in cases():
+        A, exact, E2, e1f, e2f, d, c, q32 = evaluate(D, q, p, extra, with_frame=True)
+        for lo, hi in boxes_around(rng, p, q, D):
+            lo32, hi32 = box32(lo, hi, c)
+            bd32 = walk_box_dist(q32, lo32, hi32)
+            broken += bd32 > Fraction(shrink(D) * d)
+    assert broken > 0
+
+
+def test_update_thresholds_cover_every_magnitude():
+    """T' and TB' after an insert (FP32, sqrt one ulp low) against T(U(A)) and
+    TB(U(A)) over A, e1 and E2 from the subnormal range to 10^36."""
+    rng = np.random.default_rng(13)
+    for D in (1, 2, 3, 4, 8, 16):
+        for _ in range(400):
+            A = float(np.float32(10.0 ** rng.uniform(-44, 36)))
+            e1 = f32_up(10.0 ** rng.uniform(-44, 17))
+            e2 = f32_up(10.0 ** rng.uniform(-44, 36))
+            U = U_of(D, A, e1, e2)
+            if not math.isfinite(U) or U > 1e37:
+                continue
+            Ti, TBi = after_insert(D, A, e1, e2)
+            assert Ti >= T_of(D, U, e1, e2) and TBi >= TB_of(D, U, e1), (D, A, e1, e2)
 
 
 @pytest.mark.parametrize("drop", ["E2-N2", "E2-uN2", "U-no-E2", "T-no-E2"])

@@ -22,9 +22,11 @@ query, threaded like its rayon path) on a bounded sample of the same requests
 on this host's cores: 1 warm-up, then the median and minimum of 3 repeats.
 
 Nested in the same line: the precision workload (n = 2^22 8-d Gaussian
-mixture, 10^6 queries, k = 32: FP64 vs FP32-screened traversal, with its HBM
-roofline and CPU ratio) and the plain GPU kernel workload (n = 2^20 uniform
-3-d points, 2^20 queries, k = 16, with its HBM roofline).
+mixture, 10^6 queries, k = 32: the FP64 traversal, the FP32 screen -- the tile
+walk, with FP64 recheck and visit certificate -- and the traversal's own FP32
+leaf screen, with the FP64 HBM roofline and the CPU ratios) and the plain GPU
+kernel workload (n = 2^20 uniform 3-d points, 2^20 queries, k = 16, with its
+HBM roofline).
 
 Other workloads (--workload): c2, c3 (those lines alone), c4 (256-tree
 batching), screen (the matrix-core screen alone on one 64-d tree), copy (a
@@ -220,7 +222,9 @@ def c2_line(args, steps: int, warmup: int, cpu: bool) -> dict:
 # --------------------------------------------------------------- precision mode
 def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     """Precision mode: n = 2^22, d = 8, 64-component Gaussian mixture, 10^6
-    queries, k = 32; all-FP64 leaf scan versus FP32 screen + FP64 recheck."""
+    queries, k = 32; the FP64 traversal versus the FP32 screens with FP64
+    recheck (the tile walk, which the FP32 screen flag takes for this batch,
+    and the traversal's own leaf screen), all bit-identical."""
     import torch
 
     from kdtree import forest as F
@@ -252,10 +256,17 @@ def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     F.timing_enable(False)
     same_idx = bool(torch.equal(oi64, oi32)) and bool(torch.equal(oi64, oit))
     same_bytes = bool(torch.equal(od64, od32)) and bool(torch.equal(od64, odt))
+    # Rows the tile walk answers by itself (the screen-only diagnostic leaves
+    # the rest unwritten).
+    osd = torch.full_like(od32, float("nan"))
+    osi = torch.full_like(oi32, -7)
+    F.query_device(tree, q, k, osd, osi, flags=F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_SCREEN_ONLY)
+    walk_answered = float((osi[:, 0] != -7).float().mean().item())
     d64, d32 = od64.cpu().numpy(), od32.cpu().numpy()
     rel = float(np.max(np.abs(d32 - d64) / np.maximum(np.abs(d64), 1e-300)))
     otree = OracleTree(data_h, ls)
     od, oi, ctr = otree.query(q_h[:10_000], k=k, counters=True, threads=host_threads())
+    E_ref = float(ctr[0]) / 10_000.0
     sample = 2000
     parity = bool(np.array_equal(oi[:sample], oi64[:sample].cpu().numpy())
                   and od[:sample].tobytes() == d64[:sample].tobytes())
@@ -272,7 +283,10 @@ def c3_line(args, steps: int, warmup: int, cpu: bool) -> dict:
     line = {
         "workload": "knn-gmm64-n4194304-d8-q1000000-k32-leafsize32-p2",
         "fp64": {"value": nq / s64, "ms_per_step": s64 * 1e3, "roofline": roof(t64)},
-        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3, "kernels": kernel_summary(t32)},
+        "fp32_screen": {"value": nq / s32, "ms_per_step": s32 * 1e3, "path": "tile walk: one tree walk per 64 "
+                        "leaf-ordered queries, FP32 screen with a proven margin, FP64 recheck + visit certificate",
+                        "vs_fp64": s64 / s32, "answered_by_walk": walk_answered,
+                        "reference_distance_evals_per_s": E_ref * nq / s32, "kernels": kernel_summary(t32)},
         "fp32_screen_traversal": {"value": nq / st, "ms_per_step": st * 1e3, "roofline": roof(tt)},
         "unit": "queries/s", "steps": steps, "bytes_per_query": bq, "E": E, "V": V, "X": X,
         "distance_evals_per_s_fp64": E * nq / s64,

@@ -52,8 +52,8 @@ enum kdt_error {
 #define KDT_QUERY_FP32_SCREEN 0x1u /* FP32 screen with a proven error margin, FP64
                                       recheck: identical results (p = 2, eps = 0 only;
                                       ignored for eps > 0).  ndim <= 16: batches of
-                                      8192 queries or more (rows of 1-4, 8 or 16
-                                      coordinates, k <= 32) take the tile walk (see
+                                      8192 queries or more on rows of 8 or 16
+                                      coordinates (k <= 32) take the tile walk (see
                                       KDT_QUERY_TILE_WALK), others the traversal's
                                       own leaf screen.
                                       For 32 <= ndim <= 64, p = 2, eps = 0, k <= 32:
@@ -88,8 +88,9 @@ enum kdt_error {
                                        coordinates, p = 2, eps = 0, k <= 32: the tile walk
                                        (one tree walk per 64 queries, FP32 screen of the
                                        points it reaches with a proven margin, FP64
-                                       recheck and visit certificate) for any batch size;
-                                       without this bit it is taken from 8192 queries on */
+                                       recheck and visit certificate) for any batch size
+                                       and these widths; without this bit it is taken from
+                                       8192 queries on, for rows of 8 or 16 */
 #define KDT_QUERY_NO_TILE_WALK 0x800u /* with KDT_QUERY_FP32_SCREEN: never the tile walk (the
                                           traversal's own FP32 leaf screen instead); both
                                           give the same results */

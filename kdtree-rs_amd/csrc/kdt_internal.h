@@ -188,7 +188,8 @@ struct QueryParams {
   bool allow_scratch = true;  // scratch allowed, stream-ordered or from a region (the matrix-core
                               // screen needs it); false for a pure enqueue with no region
   int walk = 0;               // tile walk for the FP32 screen (ndim <= 16): 0 = for batches of
-                              // WALK_MIN_QUERIES or more, 1 = any batch, -1 = never
+                              // WALK_MIN_QUERIES or more on rows of 8 or 16, 1 = any batch and
+                              // width, -1 = never
 };
 
 // Batches from which the FP32 screen of rows of up to 16 coordinates takes the

@@ -89,6 +89,12 @@ constexpr uint32_t WALK_CAND = 64;   // candidates per query: the recheck's 64 l
 constexpr uint32_t WALK_KMAX = 32;   // largest k on this path
 constexpr int WALK_WAVES = 4;        // independent wavefronts per workgroup
 constexpr uint32_t WALK_LP = 64;     // leaf points staged in LDS at a time
+// Rows from which the walk is taken without the explicit flag.  Below 8
+// coordinates a query's k-NN ball is small beside a tile's extent, so the
+// tile's union costs more than the per-query traversal saves: uniform 3-d
+// rows, 2^20 points and queries, k = 16: walk 9.0 ms + recheck 4.0 ms against
+// 2.6 ms for the FP64 traversal.
+constexpr uint64_t WALK_MIN_DIM = 8;
 
 // Waves per SIMD the walk is compiled for.  Rows of up to 8 coordinates: 4
 // (k <= 32: 128 VGPRs; the few values that do not fit are the list's starting
@@ -409,7 +415,7 @@ bool walk_applies(const Tree& t, const QueryParams& qp) {
   return qp.fp32_screen && qp.allow_scratch && qp.metric == M_L2 && qp.eps_factor == 1.0 &&
          qp.k <= (uint64_t)WALK_KMAX && t.rows32c && t.pn32c && t.boxes32c && t.frame && walk_width(t.d) &&
          t.scr_pc <= 1e18 && t.n < (1ull << 32) && qp.nq < (1ull << 40) &&
-         (qp.walk > 0 || (qp.walk == 0 && qp.nq >= WALK_MIN_QUERIES));
+         (qp.walk > 0 || (qp.walk == 0 && qp.nq >= WALK_MIN_QUERIES && t.d >= WALK_MIN_DIM));
 }
 
 size_t walk_scratch_bytes(const Tree& t, const QueryParams& qp) {

@@ -30,6 +30,7 @@ from ._core import load_library
 KDT_QUERY_FP32_SCREEN = 0x1
 KDT_QUERY_ASSUME_FINITE = 0x2
 KDT_QUERY_MATRIX_SCREEN = 0x8
+KDT_QUERY_SCREEN_ONLY = 0x4
 KDT_QUERY_GROUP_SHIFT = 4
 KDT_QUERY_NO_ORDER = 0x200
 KDT_QUERY_TILE_WALK = 0x400

@@ -318,9 +318,14 @@ __global__ __launch_bounds__(64 * WALK_WAVES, walk_min_waves<D>()) void k_walk(
               for (int j = 0; j < D; ++j) acc = __builtin_fmaf(m[j], pr[j], acc);
               A[i] = acc;
             }
+            // One test for the four points (most groups have no lane under
+            // its threshold; min ignores the NaN of padding rows).
+            const float amin = fminf(fminf(A[0], A[1]), fminf(A[2], A[3]));
+            if (__builtin_amdgcn_ballot_w64(amin <= T)) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-              if (i0 + i < mp) offer(A[i], o0 + i0 + i);
+              for (int i = 0; i < 4; ++i)
+                if (i0 + i < mp) offer(A[i], o0 + i0 + i);
+            }
           }
           __builtin_amdgcn_wave_barrier();
         }

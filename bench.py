@@ -562,7 +562,8 @@ def run_c4(args):
     """256 trees (n log-uniform in [1e3,1e6], d in {2,3,8,16}, float32 uniform
     promoted to float64), per-tree query counts log-uniform in [1, 1e5], k in
     {1, 8, 32}; leafsize autotuned over {8,16,32,64}, lanes per query over the
-    widths that hold k, and the matrix-core screen where it applies.  Shared
+    widths that hold k, and the matrix-core screen or the tile walk where they
+    apply.  Shared
     persistent launches (items of 8192+ queries on their own) versus one launch
     sequence per tree versus the batch plan and its HIP graph."""
     import torch
@@ -611,7 +612,8 @@ def run_c4(args):
         "autotuned_leafsize_counts": leafs, "autotuned_lanes_per_query_counts": groups, "autotune_s": tune_s,
         "parity_first_8_trees_bit_exact": parity,
         "all_items_bitwise_equal_across_modes": bool(modes_equal),
-        "matrix_screen_trees": sum(1 for tree, _, _ in items if tree.flags),
+        "matrix_screen_trees": sum(1 for tree, _, _ in items if tree.flags & F.KDT_QUERY_MATRIX_SCREEN),
+        "tile_walk_trees": sum(1 for tree, _, _ in items if tree.flags & F.KDT_QUERY_TILE_WALK),
     }
     batch.close()
     for tree, _, _ in items:

@@ -123,9 +123,10 @@ typedef struct kdt_batch_item {
   int64_t* out_idx;
   uint32_t group;  /* lanes per query for this item's tree (8, 16, 32, 64; 0 = from the
                       flags or from k), e.g. the width an autotuner picked */
-  uint32_t flags;  /* KDT_QUERY_FP32_SCREEN | KDT_QUERY_MATRIX_SCREEN for this item on top
-                      of the call's flags (other bits ignored), e.g. an autotuner's
-                      per-tree choice of the matrix-core screen; 0 = the call's */
+  uint32_t flags;  /* KDT_QUERY_FP32_SCREEN, KDT_QUERY_MATRIX_SCREEN, KDT_QUERY_TILE_WALK,
+                      KDT_QUERY_NO_TILE_WALK for this item on top of the call's flags
+                      (other bits ignored), e.g. an autotuner's per-tree choice of a
+                      screen; 0 = the call's */
 } kdt_batch_item;
 
 /* Node record exported for inspection (24 bytes, the reference's Node budget,

@@ -179,6 +179,8 @@ namespace {
 void item_flags(const kdt_batch_item& it, QueryParams& qp) {
   qp.fp32_screen = qp.fp32_screen || (it.flags & KDT_QUERY_FP32_SCREEN) != 0;
   qp.matrix_screen = qp.matrix_screen || (it.flags & KDT_QUERY_MATRIX_SCREEN) != 0;
+  if (it.flags & KDT_QUERY_TILE_WALK) qp.walk = 1;
+  if (it.flags & KDT_QUERY_NO_TILE_WALK) qp.walk = -1;
 }
 
 // A classified batch: the items that run their own launch sequence and the
@@ -204,7 +206,7 @@ int fail_msg(std::string& msg, std::string m, int code) {
 // Validates the items (the checks of kdt_query_device, in its order) and
 // splits them: items whose width has a compiled kernel go to the shared class
 // of (width, FP32 leaf screen); general p, other widths, k > 64, the
-// matrix-core screen and items of SINGLE_MIN queries or more run one launch
+// matrix-core screen, the tile walk and items of SINGLE_MIN queries or more run one launch
 // sequence each (own lanes per query, own leaf ordering, uniform kernel
 // arguments: their launch overhead is amortised, and in the shared launch
 // they ran slower).  `region` says whether scratch will come from a region
@@ -217,6 +219,7 @@ int classify(const kdt_batch_item* items, uint64_t n_items, double p, double max
   QueryParams& base = L.base;
   base.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   base.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
+  base.walk = (flags & KDT_QUERY_NO_TILE_WALK) ? -1 : (flags & KDT_QUERY_TILE_WALK) ? 1 : 0;
   base.group = requested_group(flags);
   bool per_tree = (flags & KDT_BATCH_PER_TREE_LAUNCHES) != 0;
   L.no_order = (flags & KDT_BATCH_NO_ORDER) != 0;
@@ -252,7 +255,7 @@ int classify(const kdt_batch_item* items, uint64_t n_items, double p, double max
     item_flags(it, qp);
     if (it.group) qp.group = (int)it.group;
     if (per_tree || base.metric == M_LP || w < 0 || it.k > 64 || it.n_queries >= SINGLE_MIN ||
-        screen_applies(t, qp)) {
+        screen_applies(t, qp) || walk_applies(t, qp)) {
       L.single.push_back(i);
     } else {
       const bool scr = qp.fp32_screen && base.metric == M_L2 && base.eps_factor == 1.0;  // the FP32 leaf screen: eps = 0

@@ -189,7 +189,9 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
     """Build with each leafsize, time `reps` queries of the sample with each
     group width (lanes per query that hold k), keep the fastest pair; then,
     where the matrix-core screen applies (16 <= d <= 64, k <= 32, for queries
-    with p = 2 and eps = 0), time it too and keep it in `flags` if faster.
+    with p = 2 and eps = 0), time it too and keep it in `flags` if faster; for
+    rows of 8 or 16 the same for the tile walk, kept when 1.3x faster than the
+    best so far.
     (The screen also takes 8 <= d < 16 when asked; it is not offered here: on the
     batching workload choosing it for 8-d trees lost 13-20% whether the sample
     was timed as given or repeated to the traversal's leaf-ordering size, since
@@ -240,6 +242,22 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
         timings["matrix_screen"] = ms
         if ms < best_ms:
             best.flags = sflags
+            best_ms = ms
+    if best.d in (8, 16) and k <= 32:
+        # The tile walk: kept only when clearly faster on the sample (an item
+        # that takes it leaves the shared persistent launch, which the sample
+        # timed alone does not show).
+        wflags = KDT_QUERY_FP32_SCREEN | KDT_QUERY_TILE_WALK
+        out_d, out_i = query_device(best, q, k, flags=KDT_QUERY_ASSUME_FINITE | wflags)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            query_device(best, q, k, out_d, out_i, flags=KDT_QUERY_ASSUME_FINITE | wflags)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        timings["tile_walk"] = ms
+        if ms * 1.3 < best_ms:
+            best.flags = wflags
     best.tune_ms = timings
     return best
 

@@ -457,8 +457,14 @@ struct RegList {
   __device__ __forceinline__ void write(uint64_t qi, uint32_t lane, const KArgs& a) const {
     if (lane < a.k) {
       const bool have = lane < cnt;
-      a.out_d[qi * a.k + lane] = have ? restore_d<M>(kd, a.inv_p) : INFINITY;
-      a.out_i[qi * a.k + lane] = have ? (int64_t)ki : (int64_t)a.n_points;
+      // The row offset is formed here, per query: kept opaque so that the
+      // compiler does not hoist `out + lane` out of the query loop into two
+      // 64-bit registers (at 64 VGPRs they were spilled to scratch at every
+      // wave's start: 2^20 3-d queries, 0.4 GB of scratch writes per launch).
+      uint64_t o = qi * a.k + lane;
+      asm volatile("" : "+v"(o));
+      a.out_d[o] = have ? restore_d<M>(kd, a.inv_p) : INFINITY;
+      a.out_i[o] = have ? (int64_t)ki : (int64_t)a.n_points;
     }
   }
 };
@@ -583,12 +589,16 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
       }
     } else if constexpr (DT > 0) {
       double qn64 = 0.0;
+      // A zero formed here, per query: a loop-invariant 0.0 was kept in a
+      // register pair across the query loop and spilled at 64 VGPRs.
+      double zero = 0.0;
+      asm volatile("" : "+v"(zero));
 #pragma unroll
       for (int c = 0; c < DT; ++c) {
         const double v = qg[c];
         if constexpr (!Q_IN_LDS) qr[c] = v;
         s_q[c] = v;
-        axes[c] = 0.0;
+        axes[c] = zero;
         if constexpr (SCREEN) {
           const float f = __double2float_rn(v);
           m32[c] = -2.0f * f;
@@ -661,10 +671,10 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
               if (nb * eps <= bound) node = near;
               else state = ST_POP;
             } else {
-              s_sa[sp] = cell; s_sb[sp] = 0.0; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;
+              s_sa[sp] = cell; s_sn[sp] = near; s_sm[sp] = KIND_BOXNEAR;  // (s_sb unused)
               ++sp;
               if (fb * eps <= bound) {  // enter_far, src/query.rs:329-341
-                s_sa[sp] = old; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
+                s_sa[sp] = old; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (dim << 2);
                 ++sp;
                 axes[dim] = far_axis;
                 node = far;
@@ -811,7 +821,7 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
             axes[edim] = ea;
             continue;
           }
-          eb = s_sb[sp];
+          eb = s_sb[sp];  // written for FAR and BOXFAR entries only (BOXNEAR leaves it unused)
           if (kind == KIND_FAR && !(eb * eps <= bound)) continue;
           en = s_sn[sp];
           test = true;
@@ -824,7 +834,7 @@ __device__ __forceinline__ void knn_ww(const KArgs& a, List& best, double* axes,
             node = en;
             cell = ea;
           } else {  // enter_far, src/query.rs:329-341
-            s_sa[sp] = axes[edim]; s_sb[sp] = 0.0; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
+            s_sa[sp] = axes[edim]; s_sn[sp] = 0; s_sm[sp] = KIND_RESTORE | (edim << 2);
             ++sp;
             axes[edim] = ea;
             node = en;

@@ -1,7 +1,8 @@
 """kdt_query_device_noalloc: the device query with its scratch in a fixed
 device array, called through the C ABI, gives byte for byte what
 kdt_query_device gives -- on two calls in a row, the second smaller, through
-the leaf-order sort (8192 queries or more) and through the matrix-core screen
+the leaf-order sort (8192 queries or more), through the matrix-core screen and
+through the tile walk (or the traversal screen where its buffers do not fit)
 -- rejects a query whose scratch would not fit before any launch, and checks
 finiteness first."""
 
@@ -81,6 +82,27 @@ def test_noalloc_matches_device_query_matrix_screen():
             print(f"64-d screen, {nq} queries", flush=True)
             q = torch.from_numpy(rng.standard_normal((nq, 64)).astype(np.float32).astype(np.float64)).cuda()
             compare(lib, tree, q, 10, F.KDT_QUERY_FP32_SCREEN)
+    finally:
+        torch.cuda.synchronize()
+        tree.free()
+
+
+def test_noalloc_matches_device_query_tile_walk():
+    """8-d FP32 screen: 100,000 queries take the tile walk in kdt_query_device,
+    whose per-wavefront buffers exceed the fixed array, so the no-allocation
+    call runs the traversal's FP32 screen instead; 9,000 queries fit and run
+    the walk in both.  Either way the bytes agree."""
+    import torch
+
+    from kdtree import forest as F
+
+    lib = lib_with_noalloc()
+    rng = np.random.default_rng(74)
+    tree = F.build(rng.standard_normal((150_000, 8)), 32)
+    try:
+        for nq in (100_000, 9_000):
+            print(f"8-d walk, {nq} queries", flush=True)
+            compare(lib, tree, torch.from_numpy(rng.standard_normal((nq, 8))).cuda(), 16, F.KDT_QUERY_FP32_SCREEN)
     finally:
         torch.cuda.synchronize()
         tree.free()

@@ -124,10 +124,12 @@ constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
 // and contracted between two barriers, which amortises the barrier and the
 // staging round over TS tiles.  Rows of 64 columns: 2 (5 matrix instructions
 // per tile; 2^20 points, k = 10: 65,536 queries 13.07 vs 13.48 ms, 8,192
-// queries 2.37 vs 2.57 ms, 1,024 queries 0.58 vs 0.66 ms).  Rows of 16 and 32
-// columns: 1.
+// queries 2.37 vs 2.57 ms, 1,024 queries 0.58 vs 0.66 ms).  Rows of 16
+// columns: 4 (2 matrix instructions per tile; 2^21 16-d normal rows, k = 10:
+// 65,536 queries 13.3 ms vs 14.7 with 1 and 13.9 with 2, 8,192 queries 2.27
+// vs 2.60 and 2.39 ms).  Rows of 32 columns: 1.
 #ifndef KDT_SCREEN_TS8
-#define KDT_SCREEN_TS8 1
+#define KDT_SCREEN_TS8 4
 #endif
 #ifndef KDT_SCREEN_TS16
 #define KDT_SCREEN_TS16 1
@@ -182,7 +184,9 @@ __device__ __forceinline__ void stage_load(const SArgs& a, uint32_t lt0, uint32_
 
 // LDS rows are padded by one uint4 (stride KS / 2 + 1 uint4), so the 32 lanes
 // of a half-wave reading 32 different rows hit distinct banks; tile i of the
-// step starts at rows + i 32 RS and pn + i 32.
+// step starts at rows + i 32 RS and pn + i 64: its 32 norm words, then 32 zero
+// words (written once per launch) that the lanes of half 1 read, so every
+// lane forms its norm operand from one LDS read and no select.
 template <int KS>
 __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, const uint2& pnv, uint32_t tid,
                                             uint4* rows, uint2* pn) {
@@ -191,7 +195,7 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, co
   if (tid < (uint32_t)St::SU) rows[at(tid)] = v0;
   const uint32_t u = tid + St::NT;
   if (St::PT > 1 && u < (uint32_t)St::SU) rows[at(u)] = v1;
-  if (tid < (uint32_t)St::TS * 32) pn[tid] = pnv;
+  if (tid < (uint32_t)St::TS * 32) pn[(tid / 32) * 64 + tid % 32] = pnv;
 }
 
 // One wave: 32 queries (B operand, columns) against 32-row tiles of points
@@ -254,7 +258,15 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
 // lane's list and its insert buffer (tile u, slot e at buf[(u IB + e) 64 SW]).
 // A tile's 16 values per lane are tested in four groups of four rows: a
 // group's per-element tests run only when some lane of the wave has a value
-// in it under its threshold (branch-free minima first).
+// in it under its threshold (branch-free minima first).  The minima are taken
+// over the values' bit patterns as signed integers: that order is the float
+// order for values >= +0 and puts every negative value (and -0) below 0, so
+// with t = max(bits(thr), 0), A < thr implies bits(A) < t (thr > 0: A >= 0
+// compares alike, A < 0 has bits < 0 < t; thr <= 0: A < thr is negative,
+// bits(A) < 0 = t).  The group tests therefore never miss a value under the
+// threshold, the per-element tests stay in float, and integer minima need no
+// canonicalising instruction before them (float minima of matrix results do,
+// one per operand).
 template <int KS>
 struct Screen {
   static constexpr int NS = KS / 8;  // f16x8 operands per lane half
@@ -267,6 +279,7 @@ struct Screen {
   float la[QT][KP];     // the KP smallest A seen by this lane, ascending
   uint32_t li[QT][KP];  // their tree positions
   float thr[QT];        // min(thr0, la[KP - 1] at the last merge)
+  int thri[QT];         // max(bits(thr), 0), the group tests' integer threshold
   float thr0[QT];       // the query's starting threshold (+inf without a seed)
   uint32_t cnt[QT];     // entries in this lane's buffer
   uint2* buf;
@@ -297,19 +310,22 @@ struct Screen {
       }
     }
     thr[u] = fminf(thr0[u], la[u][KP - 1]);
+    thri[u] = max(__float_as_int(thr[u]), 0);
     cnt[u] = 0;
   }
 
   // Test the 16 values of query tile u.  Element r of the accumulator is row
   // rowbase | (8 (r / 4) + r % 4) (disjoint bits, one immediate OR).
   __device__ __forceinline__ void test(int u, const floatx16& acc, uint32_t rowbase) {
-    float g[4];
+    int g[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) g[q] = fminf(fminf(acc[4 * q], acc[4 * q + 1]), fminf(acc[4 * q + 2], acc[4 * q + 3]));
-    if (!__builtin_amdgcn_ballot_w64(fminf(fminf(g[0], g[1]), fminf(g[2], g[3])) < thr[u])) return;
+    for (int q = 0; q < 4; ++q)
+      g[q] = min(min(__float_as_int(acc[4 * q]), __float_as_int(acc[4 * q + 1])),
+                 min(__float_as_int(acc[4 * q + 2]), __float_as_int(acc[4 * q + 3])));
+    if (!__builtin_amdgcn_ballot_w64(min(min(g[0], g[1]), min(g[2], g[3])) < thri[u])) return;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      if (__builtin_amdgcn_ballot_w64(g[q] < thr[u])) {
+      if (__builtin_amdgcn_ballot_w64(g[q] < thri[u])) {
 #pragma unroll
         for (int r = 4 * q; r < 4 * q + 4; ++r) {
           const float A = acc[r];
@@ -336,10 +352,10 @@ struct Screen {
   // chains), then the norms, then the tests.
   __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
     const uint4* rb = rows + j * RS + h * NS;
-    // Norm block, point side, of row j in half 0 (built with the rows).
-    // Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
-    const uint2 pw = pns[j];
-    const uint4 pnw = h == 0 ? make_uint4(0x3F803F80u, pw.x, pw.y, 0u) : make_uint4(0u, 0u, 0u, 0u);
+    // Norm block, point side, of row j in half 0 (built with the rows), zero
+    // in half 1.  Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
+    const uint2 pw = pns[j + 32 * h];
+    const uint4 pnw = make_uint4(h == 0 ? 0x3F803F80u : 0u, pw.x, pw.y, 0u);
     floatx16 acc[QT];
 #pragma unroll
     for (int u = 0; u < QT; ++u) acc[u] = floatx16{};
@@ -363,7 +379,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int RS = Stage<KS>::RS;
   constexpr int TS = Stage<KS>::TS;
   __shared__ uint4 s_rows[2][TS * 32 * RS];
-  __shared__ uint2 s_pn[2][TS * 32];
+  __shared__ uint2 s_pn[2][TS * 64];
   constexpr int QT = screen_qt<KS>();
   __shared__ uint2 s_buf[QT * insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
   const uint32_t tid = threadIdx.x, lane = tid & 63;
@@ -388,6 +404,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
       sc.li[u][e] = 0xFFFFFFFFu;
     }
     sc.thr[u] = sc.thr0[u];
+    sc.thri[u] = max(__float_as_int(sc.thr[u]), 0);
     sc.cnt[u] = 0;
   }
   sc.buf = s_buf + tid;
@@ -424,7 +441,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     if (active && (Q) < nsteps) {                                                                \
       _Pragma("unroll") for (int i_ = 0; i_ < TS; ++i_) {                                        \
         const uint32_t lt_ = (Q) * TS + i_;                                                      \
-        if (lt_ < t1) sc.tile(KDT_TILE(lt_), j, h, s_rows[B] + i_ * 32 * RS, s_pn[B] + i_ * 32); \
+        if (lt_ < t1) sc.tile(KDT_TILE(lt_), j, h, s_rows[B] + i_ * 32 * RS, s_pn[B] + i_ * 64); \
       }                                                                                          \
     }                                                                                            \
     __syncthreads();                                                                             \
@@ -433,6 +450,8 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
     uint2 xp = {}, yp = {};
     stage_load<KS>(a, 0, t1, tb, ts, tid, x0, x1, xp);
+    for (uint32_t u = tid; u < 2 * TS * 32; u += 64 * SW)  // the zero halves of both stages
+      s_pn[u / (TS * 32)][((u % (TS * 32)) / 32) * 64 + 32 + u % 32] = make_uint2(0u, 0u);
     stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
     stage_load<KS>(a, min(1u, nsteps - 1) * TS, t1, tb, ts, tid, x0, x1, xp);
     __builtin_amdgcn_sched_barrier(0);  // keep the issue order of the loop: x's loads, then y's

@@ -267,7 +267,7 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
 // threshold, the per-element tests stay in float, and integer minima need no
 // canonicalising instruction before them (float minima of matrix results do,
 // one per operand).
-template <int KS>
+template <int KS, int KL>
 struct Screen {
   static constexpr int NS = KS / 8;  // f16x8 operands per lane half
   static constexpr int RU = KS / 4, RS = RU + 1;
@@ -276,9 +276,9 @@ struct Screen {
   static constexpr uint32_t IB = insert_slots<KS>();
   f16x8 qh[QT][NS];     // -2 q^
   bf16x8 qnb[QT];       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
-  float la[QT][KP];     // the KP smallest A seen by this lane, ascending
-  uint32_t li[QT][KP];  // their tree positions
-  float thr[QT];        // min(thr0, la[KP - 1] at the last merge)
+  float la[QT][KL];     // the KL smallest A seen by this lane, ascending
+  uint32_t li[QT][KL];  // their tree positions
+  float thr[QT];        // min(thr0, la[KL - 1] at the last merge)
   int thri[QT];         // max(bits(thr), 0), the group tests' integer threshold
   float thr0[QT];       // the query's starting threshold (+inf without a seed)
   uint32_t cnt[QT];     // entries in this lane's buffer
@@ -286,7 +286,7 @@ struct Screen {
 
   __device__ __forceinline__ void insert(int u, float A, uint32_t row) {  // dropping the largest
 #pragma unroll
-    for (int e = KP - 1; e > 0; --e) {
+    for (int e = KL - 1; e > 0; --e) {
       const bool up = A < la[u][e - 1];
       const bool here = !up && A < la[u][e];
       la[u][e] = up ? la[u][e - 1] : (here ? A : la[u][e]);
@@ -306,10 +306,10 @@ struct Screen {
       if (e < cnt[u]) {
         const uint2 v = buf[(u * IB + e) * 64 * SW];
         const float A = __uint_as_float(v.x);
-        if (A < la[u][KP - 1]) insert(u, A, v.y);
+        if (A < la[u][KL - 1]) insert(u, A, v.y);
       }
     }
-    thr[u] = fminf(thr0[u], la[u][KP - 1]);
+    thr[u] = fminf(thr0[u], la[u][KL - 1]);
     thri[u] = max(__float_as_int(thr[u]), 0);
     cnt[u] = 0;
   }
@@ -373,7 +373,9 @@ struct Screen {
   }
 };
 
-template <int KS>
+// KL: entries per lane list (KP in the main pass; the seeding pass may keep
+// fewer, its lists only bound A_k from above).
+template <int KS, int KL>
 __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int SW = screen_waves<KS>();
   constexpr int RS = Stage<KS>::RS;
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   // Wave w of the workgroup takes query tiles (blockIdx.x SW + w) QT + u.
   const uint64_t qtile0 = ((uint64_t)blockIdx.x * SW + (tid >> 6)) * QT;
   const bool active = qtile0 * 32 < a.nq;  // wave-uniform: a wave past the last query only stages tiles
-  Screen<KS> sc;
+  Screen<KS, KL> sc;
   constexpr int NS = KS / 8, RQ = KS / 4;
 #pragma unroll
   for (int u = 0; u < QT; ++u) {
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     sc.qnb[u] = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
     sc.thr0[u] = a.thr0 ? a.thr0[qrow] : INFINITY;
 #pragma unroll
-    for (int e = 0; e < KP; ++e) {
+    for (int e = 0; e < KL; ++e) {
       sc.la[u][e] = INFINITY;
       sc.li[u][e] = 0xFFFFFFFFu;
     }
@@ -469,9 +471,9 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     if (active) sc.merge(u);  // wave-uniform
     const uint64_t qrow = (qtile0 + u) * 32 + j;
     if (qrow < a.nq) {
-      const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KP;
+      const size_t base = (qrow * (2 * a.slices) + 2 * blockIdx.y + h) * KL;
 #pragma unroll
-      for (int e = 0; e < KP; ++e) {
+      for (int e = 0; e < KL; ++e) {
         a.part_a[base + e] = sc.la[u][e];
         a.part_i[base + e] = sc.li[u][e];
       }
@@ -491,6 +493,7 @@ struct SelArgs {
   uint64_t nq;
   const double* __restrict__ frame;  // [D + 1] the tree's centre and sigma
   uint32_t D, k, lists;
+  uint32_t kl;               // entries per list (KP; fewer for the seeding pass's lists)
   double sigma, pc, cutoff;  // sigma, P_c >= max over rows of |p - c|, the reduced cutoff
 };
 
@@ -501,15 +504,16 @@ struct SelArgs {
 __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
-  const float* la0 = a.part_a + qi * a.lists * KP;
-  const uint32_t* li0 = a.part_i + qi * a.lists * KP;
+  const uint32_t kl = a.kl;
+  const float* la0 = a.part_a + qi * a.lists * kl;
+  const uint32_t* li0 = a.part_i + qi * a.lists * kl;
   uint32_t head[LISTS_PER_LANE];
   float hv[LISTS_PER_LANE];  // value at each head (+inf when exhausted or absent)
 #pragma unroll
   for (int c = 0; c < LISTS_PER_LANE; ++c) {
     const uint32_t L = lane + 64 * c;
     head[c] = 0;
-    hv[c] = L < a.lists ? la0[L * KP] : INFINITY;
+    hv[c] = L < a.lists ? la0[L * kl] : INFINITY;
   }
   float ak = INFINITY;
   for (uint32_t r = 0; r < a.k; ++r) {
@@ -529,7 +533,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
         if (!done && hv[c] == m) {
           done = true;
           ++head[c];
-          hv[c] = head[c] < KP ? la0[(lane + 64 * c) * KP + head[c]] : INFINITY;
+          hv[c] = head[c] < kl ? la0[(lane + 64 * c) * kl + head[c]] : INFINITY;
         }
       }
     }
@@ -588,11 +592,11 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
     const uint32_t L = lane + 64 * c;
     uint32_t n = 0;
     if (L < a.lists) {
-      const float* la = la0 + L * KP;
-      const uint32_t* li = li0 + L * KP;
-      const float last = la[KP - 1];
+      const float* la = la0 + L * kl;
+      const uint32_t* li = li0 + L * kl;
+      const float last = la[kl - 1];
       complete = complete && (last == INFINITY || last > ta);
-      while (n < KP && li[n] != 0xFFFFFFFFu && la[n] <= ta) ++n;
+      while (n < kl && li[n] != 0xFFFFFFFFu && la[n] <= ta) ++n;
     }
     cnt_c[c] = n;
     mcount += n;
@@ -611,7 +615,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
     uint32_t off = incl - mcount;
 #pragma unroll
     for (int c = 0; c < LISTS_PER_LANE; ++c) {
-      const uint32_t* li = li0 + (lane + 64 * c) * KP;
+      const uint32_t* li = li0 + (lane + 64 * c) * kl;
       for (uint32_t e = 0; e < cnt_c[c]; ++e) a.cand[qi * CAND_CAP + off + e] = li[e];
       off += cnt_c[c];
     }
@@ -830,7 +834,7 @@ uint64_t screen_resident_ks() {
   if (!cache[dev]) {
     int cus = 0, per = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_screen<KS>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(k_screen<KS, KP>),
                                                      64 * screen_waves<KS>(), 0) != hipSuccess || per <= 0)
       per = 1;
     cache[dev] = cus * per;
@@ -896,6 +900,19 @@ uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
 constexpr uint32_t SEED_STRIDE = 64;
 constexpr uint32_t SEED_MIN_TILES = KDT_SCREEN_SEED_MIN_TILES;
 bool screen_seeded(uint32_t n_tiles) { return n_tiles >= SEED_MIN_TILES; }
+
+// Lists of the seeding pass.  Its seed only has to bound A_k from above, and
+// the k-th smallest A over any set of kept entries does, however short the
+// lists: a lane that keeps its SEED_KL smallest values instead of KP gives a
+// seed no lower (a looser one only when more than SEED_KL of the sampled top
+// k fall into one lane).  Its slices interleave the sampled tiles, so a
+// query's sampled top k spread over all 2 s0 lanes: with 2 s0 SEED_KL >= 2 k
+// entries a lane rarely holds more than SEED_KL of them.  Short lists are what
+// makes the pass cheap: a lane's list starts empty, so nearly every value of
+// its first tiles goes through the insert network (about L (1 + ln(m / L))
+// inserts for m values and lists of L), and each insert moves L entries.
+constexpr uint32_t SEED_KL = 4;
+uint32_t seed_list_len(uint32_t s0, uint32_t k) { return 2 * s0 * SEED_KL >= 2 * k ? SEED_KL : (uint32_t)KP; }
 
 }  // namespace
 
@@ -979,12 +996,18 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   }
   const int sw = ks == 8 ? screen_waves<8>() : ks == 16 ? screen_waves<16>() : screen_waves<32>();
   const int qt = ks == 8 ? screen_qt<8>() : ks == 16 ? screen_qt<16>() : screen_qt<32>();
-  auto screen = [&](const SArgs& args, uint32_t ny, const char* name, double work) {
+  auto screen = [&](const SArgs& args, uint32_t ny, uint32_t kl, const char* name, double work) {
     const dim3 grid((uint32_t)((qtiles + (uint64_t)sw * qt - 1) / ((uint64_t)sw * qt)), ny);
     KernelTimer tm(s);
-    if (ks == 8) k_screen<8><<<grid, 64 * sw, 0, s>>>(args);
-    else if (ks == 16) k_screen<16><<<grid, 64 * sw, 0, s>>>(args);
-    else k_screen<32><<<grid, 64 * sw, 0, s>>>(args);
+    if (kl == SEED_KL) {
+      if (ks == 8) k_screen<8, SEED_KL><<<grid, 64 * sw, 0, s>>>(args);
+      else if (ks == 16) k_screen<16, SEED_KL><<<grid, 64 * sw, 0, s>>>(args);
+      else k_screen<32, SEED_KL><<<grid, 64 * sw, 0, s>>>(args);
+    } else {
+      if (ks == 8) k_screen<8, KP><<<grid, 64 * sw, 0, s>>>(args);
+      else if (ks == 16) k_screen<16, KP><<<grid, 64 * sw, 0, s>>>(args);
+      else k_screen<32, KP><<<grid, 64 * sw, 0, s>>>(args);
+    }
     tm.stop(name, work);
   };
 
@@ -1000,6 +1023,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   se.D = (uint32_t)t.d;
   se.k = k;
   se.lists = (uint32_t)(2 * slices);
+  se.kl = KP;
   se.frame = t.frame;
   se.sigma = t.scr_sigma;
   se.pc = t.scr_pc;
@@ -1017,15 +1041,17 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     sp.tb_mul = SEED_STRIDE;
     sp.ts = SEED_STRIDE * s0;
     sp.tmax = 0xFFFFFFFFu;
-    screen(sp, s0, "k_screen_seed", 2.0 * (double)t.d * 32.0 * (double)ns * (double)nq);
+    const uint32_t kl = seed_list_len(s0, k);
+    screen(sp, s0, kl, "k_screen_seed", 2.0 * (double)t.d * 32.0 * (double)ns * (double)nq);
     SelArgs ss = se;
     ss.lists = 2 * s0;
+    ss.kl = kl;
     ss.seed = thr0;
     k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(ss);
     sa.thr0 = thr0;
   }
   // Algorithmic work: 2 D FLOPs per (query, point) pair of the contraction.
-  screen(sa, (uint32_t)slices, ks == 8 ? "k_screen<8>" : ks == 16 ? "k_screen<16>" : "k_screen<32>",
+  screen(sa, (uint32_t)slices, KP, ks == 8 ? "k_screen<8>" : ks == 16 ? "k_screen<16>" : "k_screen<32>",
          2.0 * (double)t.d * (double)t.n * (double)nq);
 
   launch_zero_u32(nflag, 1, s);

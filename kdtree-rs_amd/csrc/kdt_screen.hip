@@ -93,7 +93,10 @@ struct SArgs {
   const uint2* __restrict__ qnorm;      // [nq] their norm-block words, query side
   float* __restrict__ part_a;        // [nq][2 S][KP] the kept A values, ascending
   uint32_t* __restrict__ part_i;     // [nq][2 S][KP] their tree positions
-  const float* __restrict__ thr0;  // [nq] starting threshold per query (k_screen_seed), or null: +inf
+  // [nq] per query: the seed (k_screen_seed), lowered during the main pass to
+  // the smallest published last entry of the query's lists (below); or null:
+  // +inf and no sharing.
+  float* __restrict__ thr0;
   uint64_t nq;
   uint32_t D, n, n_tiles, slices;
   // Slice y takes point tiles y tb_mul + i ts for i < tmax (below n_tiles).
@@ -278,10 +281,11 @@ struct Screen {
   bf16x8 qnb[QT];       // norm block, query side: (q1, q2, q3, 1, 1, 1, 0, 0) in half 0, zero in half 1
   float la[QT][KL];     // the KL smallest A seen by this lane, ascending
   uint32_t li[QT][KL];  // their tree positions
-  float thr[QT];        // min(thr0, la[KL - 1] at the last merge)
+  float thr[QT];        // min(thr0, la[KL - 1] and the query's shared threshold at the last merge)
   int thri[QT];         // max(bits(thr), 0), the group tests' integer threshold
   float thr0[QT];       // the query's starting threshold (+inf without a seed)
   uint32_t cnt[QT];     // entries in this lane's buffer
+  int* gthr[QT];        // the query's shared threshold (bits of a float >= +0), or null
   uint2* buf;
 
   __device__ __forceinline__ void insert(int u, float A, uint32_t row) {  // dropping the largest
@@ -310,6 +314,21 @@ struct Screen {
       }
     }
     thr[u] = fminf(thr0[u], la[u][KL - 1]);
+    if (gthr[u]) {
+      // Publish max(last entry, +0) to the query's shared threshold and take
+      // the smallest one published so far by any list of the query (any
+      // slice or half): a value >= some list's last entry is not among the
+      // query's KL smallest, and if it were <= tau_A that list's final last
+      // entry would be too, which the select flags.  Bits of floats >= +0
+      // order as integers.
+      // The value is read first and the atomic issued only by a lane that
+      // lowers it (hundreds of lists of a query publish when a call has few
+      // queries and many point slices); its result is not waited for.
+      const int pub = max(__float_as_int(la[u][KL - 1]), 0);
+      const int cur = __hip_atomic_load(gthr[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (pub < cur) atomicMin(gthr[u], pub);
+      thr[u] = fminf(thr[u], __int_as_float(min(cur, pub)));
+    }
     thri[u] = max(__float_as_int(thr[u]), 0);
     cnt[u] = 0;
   }
@@ -400,6 +419,9 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     const uint2 w = a.qnorm[qrow];
     sc.qnb[u] = __builtin_bit_cast(bf16x8, h == 0 ? make_uint4(w.x, w.y, 0x3F803F80u, 0u) : make_uint4(0u, 0u, 0u, 0u));
     sc.thr0[u] = a.thr0 ? a.thr0[qrow] : INFINITY;
+    // Only lanes whose lists the select checks publish (rows past nq repeat
+    // the last query and are not written).
+    sc.gthr[u] = a.thr0 && (qtile0 + u) * 32 + j < a.nq ? reinterpret_cast<int*>(a.thr0) + qrow : nullptr;
 #pragma unroll
     for (int e = 0; e < KL; ++e) {
       sc.la[u][e] = INFINITY;

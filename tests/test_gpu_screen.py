@@ -180,3 +180,43 @@ def test_screen_large_batch_device(k):
         assert gd.cpu().numpy().tobytes() == od.tobytes()
     finally:
         tree.free()
+
+
+@pytest.mark.parametrize("d,k", [(64, 10), (16, 16)])
+def test_seeded_clusters_shared_threshold(d, k):
+    """A seeded tree (>= 1024 point tiles) of tight clusters: many lists of a
+    query fill up with values under tau_A, and every list of a query lowers
+    the threshold of all the others (a value at or above any of its lists' last
+    entries is dropped).  Queries whose lists cannot certify completeness must
+    be flagged and answered by the traversal; every answer equals the
+    oracle's bit for bit, and some queries are answered by the screen itself."""
+    import torch
+
+    from kdtree import forest as F
+
+    rng = np.random.default_rng(61 + d)
+    centres = rng.uniform(-5, 5, size=(500, d))
+    scale = np.where(np.arange(500) < 250, 0.02, 0.3)[:, None]  # tight clusters: whole lists under tau_A
+    lab = rng.integers(0, 500, size=48_000)
+    data = (centres[lab] + scale[lab] * rng.standard_normal((48_000, d))).astype(np.float32).astype(np.float64)
+    ql = rng.integers(0, 500, size=1500)
+    q = np.concatenate([centres[ql] + scale[ql] * rng.standard_normal((1500, d)),
+                        rng.uniform(-5, 5, size=(500, d))]).astype(np.float32).astype(np.float64)
+    od, oi = OracleTree(data, 32).query(q, k=k, threads=8)
+    tree = F.build(data, 32)
+    try:
+        qd = torch.from_numpy(q).cuda()
+        flags = F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_MATRIX_SCREEN
+        gd, gi = F.query_device(tree, qd, k, flags=flags)
+        sd = torch.full_like(gd, -7.0)
+        si = torch.full_like(gi, -7)
+        F.query_device(tree, qd, k, sd, si, flags=flags | F.KDT_QUERY_SCREEN_ONLY)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(gi.cpu().numpy(), oi)
+        assert gd.cpu().numpy().tobytes() == od.tobytes()
+        done = (si[:, 0] != -7).cpu().numpy()
+        print(f"{d}-d clusters, k = {k}: screen answered {done.mean():.3f}", flush=True)
+        assert 0.2 < done.mean() < 1.0
+        np.testing.assert_array_equal(si.cpu().numpy()[done], oi[done])
+    finally:
+        tree.free()

@@ -936,8 +936,10 @@ inline void launch_reg(const KArgs& a, hipStream_t s) {
   // each group several queries); a larger batch loops (knn_body's next()).
   uint64_t blocks = std::min<uint64_t>((a.nq + GPB - 1) / GPB, 1u << 30);
   // Only the queries a screen left over (a.only): usually none or a few, so
-  // the groups of 16384 workgroups each look through a share of the flags.
-  if (a.only && blocks > (1u << 14)) blocks = 1u << 14;
+  // the groups of 2048 workgroups (about what the device holds at once) each
+  // look through a share of the flags; a launch with nothing flagged then
+  // starts few workgroups beside the other streams' work.
+  if (a.only && blocks > (1u << 11)) blocks = 1u << 11;
   KernelTimer tm(s);
   if (a.stk <= 32) knn_reg_kernel<DT, G, M, SCREEN, 32><<<(uint32_t)blocks, 256, 0, s>>>(a);
   else knn_reg_kernel<DT, G, M, SCREEN, STK_MAX><<<(uint32_t)blocks, 256, 0, s>>>(a);

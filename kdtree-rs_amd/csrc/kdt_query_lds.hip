@@ -9,9 +9,9 @@ int launch_knn_lds(const KArgs& a, int metric, hipStream_t s) {
   uint64_t blocks = a.nq < (1u << 20) ? a.nq : (1u << 20);
   // Only the queries a screen left over (a.only): usually none or a few, so a
   // grid of one workgroup per query would mostly start workgroups that find
-  // nothing to do; 16384 groups still give each flagged query its own group in
-  // all but extreme cases.
-  if (a.only && blocks > (1u << 14)) blocks = 1u << 14;
+  // nothing to do; 8192 single-wave groups (more than the device holds at
+  // once) each look through a share of the flags.
+  if (a.only && blocks > (1u << 13)) blocks = 1u << 13;
   KernelTimer tm(s);
   switch (metric) {
     case M_L1: knn_lds_kernel<M_L1><<<(uint32_t)blocks, 64, shm, s>>>(a); break;

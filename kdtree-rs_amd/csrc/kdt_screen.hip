@@ -599,9 +599,10 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   }
   if (a.seed) {
     // Any k points' A values bound A_k from above, so tau_A of the sampled
-    // lists' k-th value is >= the final tau_A: a lane may drop every value
-    // above it (no candidate is lost).  Out-of-range queries: no seed.
-    if (lane == 0) a.seed[qi] = qmax * sg <= 2048.0 ? ta : INFINITY;
+    // lists' k-th value is >= the final tau_A: a lane may drop every value at
+    // or above the next float up (lanes drop A >= threshold, and a point with
+    // A = tau_A can be a candidate).  Out-of-range queries: no seed.
+    if (lane == 0) a.seed[qi] = qmax * sg <= 2048.0 ? nextafterf(ta, INFINITY) : INFINITY;
     return;
   }
   // Completeness: a full list whose last value is <= tau_A may have dropped a

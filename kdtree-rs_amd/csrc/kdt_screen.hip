@@ -222,10 +222,13 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, co
 // list's 16th value as of the last merge, never below the current one, so a
 // value that skips the buffer is >= the final 16th value: after the last merge
 // each list holds the 16 smallest A of its lane's points, as with immediate
-// inserts.  IB = 16 slots for rows of <= 16 columns and 12 above (16 would
-// leave room for only one workgroup per CU beside the larger point tiles).
+// inserts.  IB = 12 slots: for rows of 16 columns (4 waves, four point tiles
+// per barrier) 40 KiB of LDS per workgroup, four per CU (16 slots: 48 KiB,
+// three per CU; 2^21 16-d rows, 8,192 queries: 1.49 vs 1.69 ms); for wider
+// rows 16 would leave room for only one workgroup per CU beside the larger
+// point tiles.
 #ifndef KDT_SCREEN_IB8
-#define KDT_SCREEN_IB8 16
+#define KDT_SCREEN_IB8 12
 #endif
 #ifndef KDT_SCREEN_IB32
 #define KDT_SCREEN_IB32 12

@@ -264,3 +264,55 @@ def test_weakened_bound_fails(weakened):
         upper = s * ((math.sqrt(max(A, 0.0) + E2) + e1) / sigma) ** 2 * (1 + 1e-12)
         broken += abs(Fraction(A) - exact) > Fraction(E2) or A > incl or d > upper
     assert broken > 0
+
+
+# --- the group tests' integer minima ---------------------------------------
+#
+# k_screen tests a group of four values against a lane's threshold thr by the
+# signed-integer minimum of their bit patterns against t = max(bits(thr), 0)
+# (an integer minimum needs no canonicalising instruction before it).  The
+# group test may pass more than needed, never less: A < thr must imply
+# bits(A) < t for every float A the accumulator can hold (any sign, zero,
+# subnormal, infinite; matrix results are never NaN) and every threshold.
+
+
+def _bits(x: np.ndarray) -> np.ndarray:
+    return np.asarray(x, np.float32).view(np.int32).astype(np.int64)
+
+
+def _special_floats() -> np.ndarray:
+    f = np.float32
+    base = [0.0, -0.0, np.inf, -np.inf, 1.0, -1.0, 2.0**-149, -(2.0**-149), 2.0**-126, -(2.0**-126),
+            np.finfo(np.float32).max, -np.finfo(np.float32).max, 1e-30, -1e-30, 3.5, -3.5]
+    rng = np.random.default_rng(77)
+    mags = 2.0 ** rng.uniform(-149, 127, size=400)
+    vals = np.concatenate([np.array(base, f), (mags * rng.choice([-1.0, 1.0], size=400)).astype(f)])
+    # neighbours one ulp apart, so that near-equal pairs are covered
+    with np.errstate(over="ignore"):  # the largest finite values step to +-inf, which are in the set anyway
+        return np.unique(np.concatenate([vals, np.nextafter(vals, f(np.inf)), np.nextafter(vals, f(-np.inf))]))
+
+
+def test_integer_group_minimum_never_misses():
+    v = _special_floats()
+    A, thr = np.meshgrid(v, v, indexing="ij")
+    A, thr = A.ravel(), thr.ravel()
+    t = np.maximum(_bits(thr), 0)
+    under = A < thr
+    assert under.any()
+    assert np.all(_bits(A)[under] < t[under])
+    # and the four-value form: min of the bit patterns under t whenever any value is under thr
+    rng = np.random.default_rng(78)
+    g = rng.choice(v, size=(20000, 4))
+    th = rng.choice(v, size=20000)
+    anyu = (g < th[:, None]).any(axis=1)
+    assert np.all(_bits(g).min(axis=1)[anyu] < np.maximum(_bits(th), 0)[anyu])
+
+
+def test_integer_group_minimum_needs_the_clamp():
+    """Without the clamp at 0 (t = bits(thr)), a negative threshold misses
+    values below it: the bit patterns of negative floats order in reverse."""
+    v = _special_floats()
+    A, thr = np.meshgrid(v, v, indexing="ij")
+    A, thr = A.ravel(), thr.ravel()
+    under = A < thr
+    assert not np.all(_bits(A)[under] < _bits(thr)[under])

@@ -125,9 +125,11 @@ constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
 
 // Point tiles per pipeline step: the TS tiles of a step are staged together
 // and contracted between two barriers, which amortises the barrier and the
-// staging round over TS tiles.  Rows of 64 columns: 2 (5 matrix instructions
-// per tile; 2^20 points, k = 10: 65,536 queries 13.07 vs 13.48 ms, 8,192
-// queries 2.37 vs 2.57 ms, 1,024 queries 0.58 vs 0.66 ms).  Rows of 16
+// staging round over TS tiles.  Rows of 64 columns: 4 (5 matrix instructions
+// per tile; 2^20 points, k = 10, 65,536 queries: 10.44 vs 10.93 ms with 2 and
+// 10.63 with 3; with 2 against 1 earlier, 13.07 vs 13.48 ms; 4 fits beside
+// two workgroups per CU with the rows loaded straight into LDS and 8 insert
+// slots, below).  Rows of 16
 // columns: 4 (2 matrix instructions per tile; 2^21 16-d normal rows, k = 10:
 // 65,536 queries 13.3 ms vs 14.7 with 1 and 13.9 with 2, 8,192 queries 2.27
 // vs 2.60 and 2.39 ms).  Rows of 32 columns: 1.
@@ -138,66 +140,80 @@ constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
 #define KDT_SCREEN_TS16 1
 #endif
 #ifndef KDT_SCREEN_TS32
-#define KDT_SCREEN_TS32 2
+#define KDT_SCREEN_TS32 4
 #endif
 template <int KS>
 constexpr int screen_tiles_per_step() { return KS <= 8 ? KDT_SCREEN_TS8 : KS <= 16 ? KDT_SCREEN_TS16 : KDT_SCREEN_TS32; }
 
 // One step: TS point tiles, each 32 rows of 2 KS fp16 (KS / 4 uint4), staged
-// global -> registers -> LDS by the whole workgroup of SW = screen_waves
-// waves: thread u copies uint4 u, u + 64 SW of the step (coalesced within each
-// tile), and threads 0 .. 32 TS - 1 the norm words.
+// by the whole workgroup of SW = screen_waves waves: thread u loads uint4 u,
+// u + 64 SW of the step straight into LDS (stage_dma), and threads
+// 0 .. 32 TS - 1 the norm words through registers.
 // (Nothing is read past a tile: TU, SU and the clamps bound every index.)
 template <int KS>
 struct Stage {
   static constexpr int RU = KS / 4;                          // uint4 per row
-  static constexpr int RS = RU + 1;                          // LDS row stride (one uint4 of padding)
   static constexpr int TU = 32 * RU;                         // uint4 per tile
   static constexpr int TS = screen_tiles_per_step<KS>();
   static constexpr int SU = TS * TU;                         // uint4 per step
   static constexpr int SW = screen_waves<KS>();
   static constexpr int NT = 64 * SW;                         // threads per workgroup
   static constexpr int PT = (SU + NT - 1) / NT;              // uint4 per thread
-  static_assert(PT <= 2, "two staging registers per thread");
+  static_assert(PT <= 2, "two loads per thread");
+  static_assert(SU % 64 == 0, "whole waves per load");
   static_assert(TS * 32 <= NT, "one thread per split norm of the step");
 };
 
-// The stage is three plain registers (v0, v1, pn), not an aggregate: an
-// aggregate carried through the loop ends up on the stack.  Loads are
-// unconditional, at clamped (in-bounds) indices: the step's local tiles
-// lt0 .. lt0 + TS - 1 are clamped to the slice's last tile t1 - 1 (a harmless
-// re-read), and local tile i is global tile tb + i ts.
+// Row layout in LDS: unpadded, TU uint4 per tile, with the uint4 columns of
+// row r stored XOR-ed with swz(r) = (r RU / 16) mod RU, so that the 16 lanes
+// of one LDS read cycle (16 rows, one column) hit distinct banks.
 template <int KS>
-__device__ __forceinline__ void stage_load(const SArgs& a, uint32_t lt0, uint32_t t1, uint32_t tb, uint32_t ts,
-                                           uint32_t tid, uint4& v0, uint4& v1, uint2& pn) {
+__device__ __forceinline__ uint32_t swz(uint32_t r) {
+  constexpr uint32_t RU = KS / 4;
+  return (r * RU / 16) % RU;
+}
+
+// The step's rows go global -> LDS directly (global_load_lds_dwordx4: lane L
+// of a wave writes 16 bytes at the wave's LDS block + 16 L): thread u of the
+// workgroup fills LDS uint4 u (and u + NT) with the source uint4 that the
+// swizzle puts there.  Loads at clamped (in-bounds) indices: the step's local
+// tiles lt0 .. lt0 + TS - 1 are clamped to the slice's last tile t1 - 1 (a
+// harmless re-read), and local tile i is global tile tb + i ts.  Completion
+// is tracked by vmcnt: the caller waits for it before the barrier that hands
+// the stage to its readers.
+template <int KS>
+__device__ __forceinline__ void stage_dma(const SArgs& a, uint32_t lt0, uint32_t t1, uint32_t tb, uint32_t ts,
+                                          uint32_t tid, uint4* rows) {
   using St = Stage<KS>;
   const uint4* src = reinterpret_cast<const uint4*>(a.rows_h);
   auto gtile = [&](uint32_t i) -> size_t { return (size_t)tb + (size_t)min(lt0 + i, t1 - 1) * ts; };
-  {
-    const uint32_t u = min(tid, (uint32_t)St::SU - 1);
-    v0 = src[gtile(u / St::TU) * St::TU + u % St::TU];
+  const uint32_t wbase = tid & ~63u;
+#pragma unroll
+  for (int p = 0; p < St::PT; ++p) {
+    const uint32_t ub = wbase + (uint32_t)p * St::NT;  // wave-uniform (SU is a multiple of 64)
+    if (ub < (uint32_t)St::SU) {
+      const uint32_t u = tid + (uint32_t)p * St::NT;
+      const uint32_t w = u % St::TU, r = w / St::RU, x = w % St::RU;
+      const uint4* g = src + gtile(u / St::TU) * St::TU + r * St::RU + (x ^ swz<KS>(r));
+      __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)(rows + ub), 16, 0, 0);
+    }
   }
-  if (St::PT > 1) {
-    const uint32_t u = min(tid + (uint32_t)St::NT, (uint32_t)St::SU - 1);
-    v1 = src[gtile(u / St::TU) * St::TU + u % St::TU];
-  }
-  const uint32_t pi = min(tid, (uint32_t)St::TS * 32 - 1);
-  pn = a.pnorm_h[gtile(pi / 32) * 32 + pi % 32];
 }
 
-// LDS rows are padded by one uint4 (stride KS / 2 + 1 uint4), so the 32 lanes
-// of a half-wave reading 32 different rows hit distinct banks; tile i of the
-// step starts at rows + i 32 RS and pn + i 64: its 32 norm words, then 32 zero
-// words (written once per launch) that the lanes of half 1 read, so every
-// lane forms its norm operand from one LDS read and no select.
 template <int KS>
-__device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, const uint2& pnv, uint32_t tid,
-                                            uint4* rows, uint2* pn) {
+__device__ __forceinline__ void pn_load(const SArgs& a, uint32_t lt0, uint32_t t1, uint32_t tb, uint32_t ts,
+                                        uint32_t tid, uint2& pn) {
   using St = Stage<KS>;
-  auto at = [](uint32_t u) { return (u / St::TU) * 32 * St::RS + ((u % St::TU) / St::RU) * St::RS + u % St::RU; };
-  if (tid < (uint32_t)St::SU) rows[at(tid)] = v0;
-  const uint32_t u = tid + St::NT;
-  if (St::PT > 1 && u < (uint32_t)St::SU) rows[at(u)] = v1;
+  const uint32_t pi = min(tid, (uint32_t)St::TS * 32 - 1);
+  pn = a.pnorm_h[((size_t)tb + (size_t)min(lt0 + pi / 32, t1 - 1) * ts) * 32 + pi % 32];
+}
+
+// Tile i of the step starts at rows + i TU and pn + i 64: its 32 norm words,
+// then 32 zero words (written once per launch) that the lanes of half 1
+// read, so every lane forms its norm operand from one LDS read and no select.
+template <int KS>
+__device__ __forceinline__ void pn_store(const uint2& pnv, uint32_t tid, uint2* pn) {
+  using St = Stage<KS>;
   if (tid < (uint32_t)St::TS * 32) pn[(tid / 32) * 64 + tid % 32] = pnv;
 }
 
@@ -215,28 +231,35 @@ __device__ __forceinline__ void stage_store(const uint4& v0, const uint4& v1, co
 // (r % 4), r = 0..15.
 //
 // A value under the lane's threshold is not inserted at once: it is appended
-// to the lane's buffer of IB slots in LDS, and after every CHK elements, if any
-// lane of the wave holds more than IB - CHK entries, all lanes merge their
+// to the lane's buffer of IB slots in LDS, and after every CHK elements (8, or
+// 4 for rows of 64 columns), if any lane of the wave holds more than IB - CHK
+// entries, all lanes merge their
 // buffers into their sorted lists together (the insert network then runs once
 // for many lanes, not once for each lane that inserts).  The threshold is the
 // list's 16th value as of the last merge, never below the current one, so a
 // value that skips the buffer is >= the final 16th value: after the last merge
 // each list holds the 16 smallest A of its lane's points, as with immediate
-// inserts.  IB = 12 slots: for rows of 16 columns (4 waves, four point tiles
-// per barrier) 40 KiB of LDS per workgroup, four per CU (16 slots: 48 KiB,
-// three per CU; 2^21 16-d rows, 8,192 queries: 1.49 vs 1.69 ms); for wider
-// rows 16 would leave room for only one workgroup per CU beside the larger
-// point tiles.
+// inserts.  IB = 12 slots for rows of 16 and 32 columns: for 16 columns (4
+// waves, four point tiles per barrier) 40 KiB of LDS per workgroup, four per
+// CU (16 slots: 48 KiB, three per CU; 2^21 16-d rows, 8,192 queries: 1.49 vs
+// 1.69 ms).  Rows of 64 columns: 8 slots with a check after every group of
+// four rows (CHK = 4; merges start at the same fill, more than 4 entries), so
+// that four point tiles per barrier (32 KiB of rows) and the buffers (32 KiB)
+// leave room for two workgroups per CU.
 #ifndef KDT_SCREEN_IB8
 #define KDT_SCREEN_IB8 12
 #endif
 #ifndef KDT_SCREEN_IB32
-#define KDT_SCREEN_IB32 12
+#define KDT_SCREEN_IB32 8
+#endif
+#ifndef KDT_SCREEN_CHK32
+#define KDT_SCREEN_CHK32 4
 #endif
 template <int KS>
 constexpr uint32_t insert_slots() { return KS <= 8 ? KDT_SCREEN_IB8 : KS <= 16 ? 12 : KDT_SCREEN_IB32; }
-constexpr int CHK = 8;  // (Screen::tile checks after every two groups of four rows)
-static_assert(KDT_SCREEN_IB8 > CHK && KDT_SCREEN_IB32 > CHK, "a buffer holds more than one check interval");
+template <int KS>
+constexpr int chk() { return KS >= 32 ? KDT_SCREEN_CHK32 : 8; }
+static_assert(KDT_SCREEN_IB8 > 8 && KDT_SCREEN_IB32 > KDT_SCREEN_CHK32, "a buffer holds more than one check interval");
 
 // The queries in the screen's operand form, once per call (every point
 // slice reads them): row i as -2 q^ (q^_j = screen_f16(fl64(q_j - c_j),
@@ -276,7 +299,7 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
 template <int KS, int KL>
 struct Screen {
   static constexpr int NS = KS / 8;  // f16x8 operands per lane half
-  static constexpr int RU = KS / 4, RS = RU + 1;
+  static constexpr int RU = KS / 4;
   static constexpr int QT = screen_qt<KS>();
   static constexpr int SW = screen_waves<KS>();
   static constexpr uint32_t IB = insert_slots<KS>();
@@ -364,8 +387,10 @@ struct Screen {
           }
         }
       }
-      // cnt <= IB - CHK before each pair of groups (CHK = 8 elements), so <= IB after it.
-      if (q % 2 == 1 && __builtin_amdgcn_ballot_w64(cnt[u] > IB - CHK)) merge(u);
+      // cnt <= IB - CHK before each check interval of CHK elements (one or two
+      // groups), so <= IB after it.
+      constexpr int CHK = chk<KS>();
+      if ((CHK == 4 || q % 2 == 1) && __builtin_amdgcn_ballot_w64(cnt[u] > IB - CHK)) merge(u);
     }
   }
 
@@ -373,7 +398,8 @@ struct Screen {
   // from the NS fp16 products (each point operand, read once, feeds the QT
   // chains), then the norms, then the tests.
   __device__ __forceinline__ void tile(uint32_t t, uint32_t j, uint32_t h, const uint4* rows, const uint2* pns) {
-    const uint4* rb = rows + j * RS + h * NS;
+    const uint4* rb = rows + j * RU;
+    const uint32_t sj = swz<KS>(j);
     // Norm block, point side, of row j in half 0 (built with the rows), zero
     // in half 1.  Padding rows: p1 = +inf, p2 = p3 = 0, A = +inf.
     const uint2 pw = pns[j + 32 * h];
@@ -383,7 +409,7 @@ struct Screen {
     for (int u = 0; u < QT; ++u) acc[u] = floatx16{};
 #pragma unroll
     for (int v = 0; v < NS; ++v) {
-      const f16x8 pv = __builtin_bit_cast(f16x8, rb[v]);
+      const f16x8 pv = __builtin_bit_cast(f16x8, rb[(h * NS + v) ^ sj]);
 #pragma unroll
       for (int u = 0; u < QT; ++u) acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(pv, qh[u][v], acc[u], 0, 0, 0);
     }
@@ -400,9 +426,9 @@ struct Screen {
 template <int KS, int KL>
 __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   constexpr int SW = screen_waves<KS>();
-  constexpr int RS = Stage<KS>::RS;
   constexpr int TS = Stage<KS>::TS;
-  __shared__ uint4 s_rows[2][TS * 32 * RS];
+  constexpr int RU = Stage<KS>::RU;
+  __shared__ uint4 s_rows[2][TS * 32 * RU];
   __shared__ uint2 s_pn[2][TS * 64];
   constexpr int QT = screen_qt<KS>();
   __shared__ uint2 s_buf[QT * insert_slots<KS>() * 64 * SW];  // per-lane insert buffers, slot-major
@@ -461,32 +487,36 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
   // (A macro rather than a lambda: the two register stages must stay named
   // registers, never a stack object behind a reference.)
   const uint32_t nsteps = (t1 + TS - 1) / TS;
-#define KDT_SCREEN_STEP(Q, B, N0, N1, NP)                                                        \
+#define KDT_SCREEN_STEP(Q, B, NP)                                                                \
   do {                                                                                           \
-    if ((Q) + 1 < nsteps) stage_store<KS>(N0, N1, NP, tid, s_rows[(B) ^ 1], s_pn[(B) ^ 1]);      \
-    stage_load<KS>(a, min((Q) + 3, nsteps - 1) * TS, t1, tb, ts, tid, N0, N1, NP);              \
+    if ((Q) + 1 < nsteps) {                                                                      \
+      pn_store<KS>(NP, tid, s_pn[(B) ^ 1]);                                                      \
+      stage_dma<KS>(a, ((Q) + 1) * TS, t1, tb, ts, tid, s_rows[(B) ^ 1]);                        \
+    }                                                                                            \
+    pn_load<KS>(a, min((Q) + 3, nsteps - 1) * TS, t1, tb, ts, tid, NP);                          \
     if (active && (Q) < nsteps) {                                                                \
       _Pragma("unroll") for (int i_ = 0; i_ < TS; ++i_) {                                        \
         const uint32_t lt_ = (Q) * TS + i_;                                                      \
-        if (lt_ < t1) sc.tile(KDT_TILE(lt_), j, h, s_rows[B] + i_ * 32 * RS, s_pn[B] + i_ * 64); \
+        if (lt_ < t1) sc.tile(KDT_TILE(lt_), j, h, s_rows[B] + i_ * 32 * RU, s_pn[B] + i_ * 64); \
       }                                                                                          \
     }                                                                                            \
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");                                             \
     __syncthreads();                                                                             \
   } while (0)
   if (nsteps > 0) {  // uniform over the workgroup
-    uint4 x0 = {}, x1 = {}, y0 = {}, y1 = {};
     uint2 xp = {}, yp = {};
-    stage_load<KS>(a, 0, t1, tb, ts, tid, x0, x1, xp);
+    stage_dma<KS>(a, 0, t1, tb, ts, tid, s_rows[0]);
+    pn_load<KS>(a, 0, t1, tb, ts, tid, xp);
     for (uint32_t u = tid; u < 2 * TS * 32; u += 64 * SW)  // the zero halves of both stages
       s_pn[u / (TS * 32)][((u % (TS * 32)) / 32) * 64 + 32 + u % 32] = make_uint2(0u, 0u);
-    stage_store<KS>(x0, x1, xp, tid, s_rows[0], s_pn[0]);
-    stage_load<KS>(a, min(1u, nsteps - 1) * TS, t1, tb, ts, tid, x0, x1, xp);
-    __builtin_amdgcn_sched_barrier(0);  // keep the issue order of the loop: x's loads, then y's
-    stage_load<KS>(a, min(2u, nsteps - 1) * TS, t1, tb, ts, tid, y0, y1, yp);
+    pn_store<KS>(xp, tid, s_pn[0]);
+    pn_load<KS>(a, min(1u, nsteps - 1) * TS, t1, tb, ts, tid, xp);
+    pn_load<KS>(a, min(2u, nsteps - 1) * TS, t1, tb, ts, tid, yp);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     for (uint32_t q = 0; q < nsteps; q += 2) {
-      KDT_SCREEN_STEP(q, 0, x0, x1, xp);
-      KDT_SCREEN_STEP(q + 1, 1, y0, y1, yp);
+      KDT_SCREEN_STEP(q, 0, xp);
+      KDT_SCREEN_STEP(q + 1, 1, yp);
     }
   }
 #undef KDT_SCREEN_STEP

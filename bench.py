@@ -614,6 +614,9 @@ def run_c4(args):
         "all_items_bitwise_equal_across_modes": bool(modes_equal),
         "matrix_screen_trees": sum(1 for tree, _, _ in items if tree.flags & F.KDT_QUERY_MATRIX_SCREEN),
         "tile_walk_trees": sum(1 for tree, _, _ in items if tree.flags & F.KDT_QUERY_TILE_WALK),
+        "screen_slice_codes": {str(c): sum(1 for tree, _, _ in items if tree.flags & F.KDT_QUERY_MATRIX_SCREEN
+                                           and (tree.flags & F.KDT_QUERY_SLICES_MASK) >> F.KDT_QUERY_SLICES_SHIFT == c)
+                               for c in range(4)},
     }
     batch.close()
     for tree, _, _ in items:

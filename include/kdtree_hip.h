@@ -94,6 +94,14 @@ enum kdt_error {
 #define KDT_QUERY_NO_TILE_WALK 0x800u /* with KDT_QUERY_FP32_SCREEN: never the tile walk (the
                                           traversal's own FP32 leaf screen instead); both
                                           give the same results */
+#define KDT_QUERY_SLICES_SHIFT 12u /* bits 12-13, with the matrix-core screen: its point
+                                      slices (the range of point tiles one workgroup
+                                      sweeps for its block of queries, i.e. the work
+                                      tile's shape) relative to its occupancy rule:
+                                      0 = the rule, 1 = half as many, 2 = twice, 3 = four
+                                      times; a per-tree choice, made by timing
+                                      (kdtree.forest); results are the same */
+#define KDT_QUERY_SLICES_MASK 0x3000u
 #define KDT_QUERY_NO_ORDER 0x200u /* keep the given query order: batches of 8192 or more
                                      queries otherwise run grouped by leaf (a leaf-order
                                      sort with stream-ordered scratch); results are the
@@ -124,9 +132,9 @@ typedef struct kdt_batch_item {
   uint32_t group;  /* lanes per query for this item's tree (8, 16, 32, 64; 0 = from the
                       flags or from k), e.g. the width an autotuner picked */
   uint32_t flags;  /* KDT_QUERY_FP32_SCREEN, KDT_QUERY_MATRIX_SCREEN, KDT_QUERY_TILE_WALK,
-                      KDT_QUERY_NO_TILE_WALK for this item on top of the call's flags
-                      (other bits ignored), e.g. an autotuner's per-tree choice of a
-                      screen; 0 = the call's */
+                      KDT_QUERY_NO_TILE_WALK, KDT_QUERY_SLICES_* for this item on top of
+                      the call's flags (other bits ignored), e.g. an autotuner's per-tree
+                      choice of a screen and its work-tile shape; 0 = the call's */
 } kdt_batch_item;
 
 /* Node record exported for inspection (24 bytes, the reference's Node budget,

@@ -234,6 +234,7 @@ int kdt::query_args(const kdt_tree* tree, const double* q, uint64_t nq, uint64_t
   qp.group = kdt::requested_group(flags);
   qp.allow_order = (flags & KDT_QUERY_NO_ORDER) == 0;
   qp.walk = (flags & KDT_QUERY_NO_TILE_WALK) ? -1 : (flags & KDT_QUERY_TILE_WALK) ? 1 : 0;
+  qp.slice_shift = kdt::slice_shift_of(flags);
   scalar_rc = kdt::query_params(p, max_distance, eps, qp, smsg);
   return KDT_OK;
 }

@@ -181,6 +181,7 @@ void item_flags(const kdt_batch_item& it, QueryParams& qp) {
   qp.matrix_screen = qp.matrix_screen || (it.flags & KDT_QUERY_MATRIX_SCREEN) != 0;
   if (it.flags & KDT_QUERY_TILE_WALK) qp.walk = 1;
   if (it.flags & KDT_QUERY_NO_TILE_WALK) qp.walk = -1;
+  if (it.flags & KDT_QUERY_SLICES_MASK) qp.slice_shift = slice_shift_of(it.flags);
 }
 
 // A classified batch: the items that run their own launch sequence and the
@@ -220,6 +221,7 @@ int classify(const kdt_batch_item* items, uint64_t n_items, double p, double max
   base.fp32_screen = (flags & KDT_QUERY_FP32_SCREEN) != 0;
   base.matrix_screen = (flags & KDT_QUERY_MATRIX_SCREEN) != 0;
   base.walk = (flags & KDT_QUERY_NO_TILE_WALK) ? -1 : (flags & KDT_QUERY_TILE_WALK) ? 1 : 0;
+  base.slice_shift = slice_shift_of(flags);
   base.group = requested_group(flags);
   bool per_tree = (flags & KDT_BATCH_PER_TREE_LAUNCHES) != 0;
   L.no_order = (flags & KDT_BATCH_NO_ORDER) != 0;

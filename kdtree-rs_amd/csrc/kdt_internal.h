@@ -41,6 +41,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/kdtree_hip.h"
+
 namespace kdt {
 
 constexpr uint32_t LEAF_BIT = 0x80000000u;
@@ -187,6 +189,7 @@ struct QueryParams {
   bool allow_order = true;    // large batches may run in leaf order
   bool allow_scratch = true;  // scratch allowed, stream-ordered or from a region (the matrix-core
                               // screen needs it); false for a pure enqueue with no region
+  int slice_shift = 0;        // KDT_QUERY_SLICES_*: matrix-core screen slices x 2^slice_shift (-1 .. 2)
   int walk = 0;               // tile walk for the FP32 screen (ndim <= 16): 0 = for batches of
                               // WALK_MIN_QUERIES or more on rows of 8 or 16, 1 = any batch and
                               // width, -1 = never
@@ -195,6 +198,11 @@ struct QueryParams {
 // Batches from which the FP32 screen of rows of up to 16 coordinates takes the
 // tile walk (kdt_walk.hip) rather than the traversal's own leaf screen.
 constexpr uint64_t WALK_MIN_QUERIES = 8192;
+// The slice scale of KDT_QUERY_SLICES_*: 0 (the rule), -1 (half), 1, 2.
+inline int slice_shift_of(uint32_t flags) {
+  const uint32_t c = (flags & KDT_QUERY_SLICES_MASK) >> KDT_QUERY_SLICES_SHIFT;
+  return c == 1 ? -1 : (int)(c == 0 ? 0 : c - 1);
+}
 
 // Whether a batch runs in leaf order (kdt_query.hip): large enough and allowed.
 bool order_applies(const QueryParams& qp);

@@ -938,6 +938,12 @@ uint64_t screen_slices(const Tree& t, const QueryParams& qp, uint32_t* per) {
       best_per = p;
     }
   }
+  if (qp.slice_shift != 0) {  // KDT_QUERY_SLICES_*: a per-tree work-tile shape
+    uint64_t S = qp.slice_shift > 0 ? best_s << qp.slice_shift : (best_s + 1) / 2;
+    S = std::min(std::max(S, lo), hi);
+    best_per = (n_tiles + S - 1) / S;
+    best_s = (n_tiles + best_per - 1) / best_per;
+  }
   *per = (uint32_t)best_per;
   return best_s;
 }

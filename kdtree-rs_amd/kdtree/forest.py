@@ -35,6 +35,8 @@ KDT_QUERY_GROUP_SHIFT = 4
 KDT_QUERY_NO_ORDER = 0x200
 KDT_QUERY_TILE_WALK = 0x400
 KDT_QUERY_NO_TILE_WALK = 0x800
+KDT_QUERY_SLICES_SHIFT = 12
+KDT_QUERY_SLICES_MASK = 0x3000
 KDT_BATCH_PER_TREE_LAUNCHES = 0x100
 KDT_BATCH_NO_ORDER = 0x200
 
@@ -189,9 +191,11 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
     """Build with each leafsize, time `reps` queries of the sample with each
     group width (lanes per query that hold k), keep the fastest pair; then,
     where the matrix-core screen applies (16 <= d <= 64, k <= 32, for queries
-    with p = 2 and eps = 0), time it too and keep it in `flags` if faster; for
-    rows of 8 or 16 the same for the tile walk, kept when 1.3x faster than the
-    best so far.
+    with p = 2 and eps = 0), time it too and keep it in `flags` if faster, then
+    its work-tile shape: the point slices per block of queries at half, twice
+    and four times its occupancy rule (KDT_QUERY_SLICES_*), one kept when 5%
+    faster than the rule; for rows of 8 or 16 the same for the tile walk, kept
+    when 1.3x faster than the best so far.
     (The screen also takes 8 <= d < 16 when asked; it is not offered here: on the
     batching workload choosing it for 8-d trees lost 13-20% whether the sample
     was timed as given or repeated to the traversal's leaf-ordering size, since
@@ -243,6 +247,22 @@ def autotune_build(data, sample_queries, k: int, candidates=(8, 16, 32, 64), gro
         if ms < best_ms:
             best.flags = sflags
             best_ms = ms
+            shape_ms, chosen = ms, sflags
+            best.flags = 0  # query_device adds a tree's flags to the call's
+            for code in (1, 2, 3):  # half, twice, four times the rule's slices
+                f = sflags | (code << KDT_QUERY_SLICES_SHIFT)
+                query_device(best, q, k, out_d, out_i, flags=KDT_QUERY_ASSUME_FINITE | f)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    query_device(best, q, k, out_d, out_i, flags=KDT_QUERY_ASSUME_FINITE | f)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) * 1e3 / reps
+                timings[f"matrix_screen_slices_{code}"] = ms
+                if ms * 1.05 < shape_ms:
+                    chosen, shape_ms = f, ms
+            best.flags = chosen
+            best_ms = min(best_ms, shape_ms)
     if best.d in (8, 16) and k <= 32:
         # The tile walk: kept only when clearly faster on the sample (an item
         # that takes it leaves the shared persistent launch, which the sample

@@ -220,3 +220,30 @@ def test_seeded_clusters_shared_threshold(d, k):
         np.testing.assert_array_equal(si.cpu().numpy()[done], oi[done])
     finally:
         tree.free()
+
+
+@pytest.mark.parametrize("d,k", [(64, 10), (16, 32)])
+def test_screen_slice_shapes_same_results(d, k):
+    """KDT_QUERY_SLICES_*: the matrix-core screen's point slices at the
+    occupancy rule, half, twice and four times it (the per-tree work-tile
+    shape the autotuner picks) all give the oracle's answer bit for bit."""
+    import torch
+
+    from kdtree import forest as F
+
+    data = f32_normal(91 + d, 40_000, d)  # 1,250 point tiles: seeded
+    q = f32_normal(92 + d, 3000, d)
+    od, oi = OracleTree(data, 32).query(q, k=k, threads=8)
+    tree = F.build(data, 32)
+    try:
+        qd = torch.from_numpy(q).cuda()
+        for code in range(4):
+            flags = (F.KDT_QUERY_ASSUME_FINITE | F.KDT_QUERY_FP32_SCREEN | F.KDT_QUERY_MATRIX_SCREEN
+                     | (code << F.KDT_QUERY_SLICES_SHIFT))
+            gd, gi = F.query_device(tree, qd, k, flags=flags)
+            torch.cuda.synchronize()
+            print(f"{d}-d k = {k}, slice code {code}", flush=True)
+            np.testing.assert_array_equal(gi.cpu().numpy(), oi)
+            assert gd.cpu().numpy().tobytes() == od.tobytes()
+    finally:
+        tree.free()

@@ -97,6 +97,10 @@ struct SArgs {
   // the smallest published last entry of the query's lists (below); or null:
   // +inf and no sharing.
   float* __restrict__ thr0;
+  // [nq] the queries' (e1, E2) (k_screen_queries): a list's k-th entry gives
+  // a bound of tau_A at each merge (screen_tau_a32); k = 0 for none.
+  const float2* __restrict__ qterm;
+  uint32_t k;
   uint64_t nq;
   uint32_t D, n, n_tiles, slices;
   // Slice y takes point tiles y tb_mul + i ts for i < tmax (below n_tiles).
@@ -261,25 +265,70 @@ template <int KS>
 constexpr int chk() { return KS >= 32 ? KDT_SCREEN_CHK32 : 8; }
 static_assert(KDT_SCREEN_IB8 > 8 && KDT_SCREEN_IB32 > KDT_SCREEN_CHK32, "a buffer holds more than one check interval");
 
+// x rounded up to float (a bound stays a bound).
+__device__ __forceinline__ float round_up_f(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = nextafterf(f, INFINITY);
+  return f;
+}
+
+// The screen's error terms for one query (the bound in the file header),
+// from qc2 = the sum of fl64(q_j - c_j)^2 in any order (|q - c| is rounded up
+// by a relative 2^-44, as P_c in the build): e1 >= | |q^ - p^| - sigma |q - p| |
+// and E2 >= |A - |q^ - p^|^2| for every row p.  Each is rounded up by a
+// relative 1e-12 (far above the FP64 error of computing it), then to float.
+__device__ __forceinline__ float2 screen_terms(double qc2, uint32_t D_, double sg, double pc) {
+  const double D = (double)D_;
+  const double qc = sqrt(qc2) * (1.0 + 0x1p-44);
+  const double ea = 0x1p-11 * (1.0 + 0x1p-12);
+  const double eb = 0x1p-14 * (1.0 + 0x1p-60) * 2.0 * sqrt(D) * (1.0 + 0x1p-50);
+  const double e1 = (ea * sg * (qc + pc) + eb) * (1.0 + 1e-12);
+  const double Nh = (sg * (qc + pc) * (1.0 + ea) + eb) * (1.0 + 1e-12);
+  const double m = (D + 5.0) * 0x1p-23;
+  const double gam = m / (1.0 - m);
+  const double E2 = (((1.0 + 0x1p-7) * gam + 1.13 * 0x1p-24) * Nh * Nh + 0x1p-118) * (1.0 + 1e-12);
+  return make_float2(round_up_f(e1), round_up_f(E2));
+}
+
+// A float strictly above the tau_A that k_screen_select derives from an
+// upper bound ak of A_k (with no cutoff, which only lowers it; e1, E2 as
+// screen_terms gives them, >= the select's): that value is
+//   T = ((s' (sqrt(ak + E2) + e1) + e1)^2 + E2) (1 + 1e-12),  s' <= 1 + 2^-44,
+// rounded up to float (one ulp, <= 2^-23 relative), so T <= (w^2 + E2)
+// (1 + 2^-22) with w = sqrt(max(ak + E2, 0)) + 2 e1.  Here the seven FP32
+// operations each err by at most 2^-23 relatively (sqrtf within one ulp, the
+// rest within half an ulp; E2 >= 2^-118 keeps the radicand's rounding, when
+// it is subnormal, far below e1's share of w), so the result is >= (w^2 +
+// E2)(1 + 2^-18)(1 - 2^-20) > (w^2 + E2)(1 + 2^-22).  Overflow gives +inf.
+__device__ __forceinline__ float screen_tau_a32(float ak, float2 terms) {
+  const float r = sqrtf(fmaxf(ak + terms.y, 0.0f)) + terms.x;
+  const float w = r + terms.x;
+  return (w * w + terms.y) * (1.0f + 0x1p-18f);
+}
+
 // The queries in the screen's operand form, once per call (every point
 // slice reads them): row i as -2 q^ (q^_j = screen_f16(fl64(q_j - c_j),
 // sigma), exact in fp16) over 2 KS columns (zero past D), and the norm block's
 // query-side words of qn = fl32(|q^|^2) (summed in FP64): (q1 | q2 << 16,
-// q3 | 1 << 16), from the three bf16 pieces of screen_norm_words.
+// q3 | 1 << 16), from the three bf16 pieces of screen_norm_words; and the
+// query's error terms (e1, E2) of screen_terms, for the main pass's bounds.
 template <int KS>
 __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint32_t D,
-                                 const double* __restrict__ frame, uint16_t* __restrict__ qrows,
-                                 uint2* __restrict__ qnorm) {
+                                 const double* __restrict__ frame, double pc, uint16_t* __restrict__ qrows,
+                                 uint2* __restrict__ qnorm, float2* __restrict__ qterm) {
   const double sigma = frame[D];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) {
-    double qn = 0.0;
+    double qn = 0.0, qc2 = 0.0;
     for (uint32_t c = 0; c < 2 * KS; ++c) {
-      const _Float16 v = c < D ? screen_f16(q[i * D + c] - frame[c], sigma) : (_Float16)0.0f;
+      const double y = c < D ? q[i * D + c] - frame[c] : 0.0;
+      const _Float16 v = c < D ? screen_f16(y, sigma) : (_Float16)0.0f;
       qrows[i * 2 * KS + c] = __builtin_bit_cast(uint16_t, (_Float16)(-2.0f * (float)v));
       qn += (double)v * (double)v;
+      qc2 += y * y;
     }
     const uint2 w = screen_norm_words((float)qn);
     qnorm[i] = make_uint2((w.x >> 16) | (w.y << 16), (w.y >> 16) | 0x3F800000u);
+    qterm[i] = screen_terms(qc2, D, sigma, pc);
   }
 }
 
@@ -312,6 +361,8 @@ struct Screen {
   float thr0[QT];       // the query's starting threshold (+inf without a seed)
   uint32_t cnt[QT];     // entries in this lane's buffer
   int* gthr[QT];        // the query's shared threshold (bits of a float >= +0), or null
+  float2 term[QT];      // the query's (e1, E2)
+  uint32_t kk;          // k when a list of KL holds a k-th entry, else 0 (wave-uniform)
   uint2* buf;
 
   __device__ __forceinline__ void insert(int u, float A, uint32_t row) {  // dropping the largest
@@ -339,18 +390,29 @@ struct Screen {
         if (A < la[u][KL - 1]) insert(u, A, v.y);
       }
     }
-    thr[u] = fminf(thr0[u], la[u][KL - 1]);
+    // The list's k-th entry ak is the A of a point, so A_k <= ak and the
+    // select's tau_A is below screen_tau_a32(ak): values at or above that are
+    // no candidates.  (The k-th entry as the largest of the first k: an index
+    // k - 1 into the list would put the list in scratch memory.)
+    float lim = la[u][KL - 1];
+    if (kk) {
+      float ak = -INFINITY;
+#pragma unroll
+      for (int e = 0; e < KL; ++e) ak = (uint32_t)e < kk ? fmaxf(ak, la[u][e]) : ak;
+      lim = fminf(lim, screen_tau_a32(ak, term[u]));
+    }
+    thr[u] = fminf(thr0[u], lim);
     if (gthr[u]) {
-      // Publish max(last entry, +0) to the query's shared threshold and take
-      // the smallest one published so far by any list of the query (any
-      // slice or half): a value >= some list's last entry is not among the
-      // query's KL smallest, and if it were <= tau_A that list's final last
-      // entry would be too, which the select flags.  Bits of floats >= +0
-      // order as integers.
+      // Publish max(lim, +0) to the query's shared threshold and take the
+      // smallest one published so far by any list of the query (any slice or
+      // half): a value >= some list's last entry is not among the query's KL
+      // smallest, and if it were <= tau_A that list's final last entry would
+      // be too, which the select flags; a value >= the other bound is no
+      // candidate.  Bits of floats >= +0 order as integers.
       // The value is read first and the atomic issued only by a lane that
       // lowers it (hundreds of lists of a query publish when a call has few
       // queries and many point slices); its result is not waited for.
-      const int pub = max(__float_as_int(la[u][KL - 1]), 0);
+      const int pub = max(__float_as_int(lim), 0);
       const int cur = __hip_atomic_load(gthr[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (pub < cur) atomicMin(gthr[u], pub);
       thr[u] = fminf(thr[u], __int_as_float(min(cur, pub)));
@@ -451,6 +513,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     // Only lanes whose lists the select checks publish (rows past nq repeat
     // the last query and are not written).
     sc.gthr[u] = a.thr0 && (qtile0 + u) * 32 + j < a.nq ? reinterpret_cast<int*>(a.thr0) + qrow : nullptr;
+    sc.term[u] = a.qterm ? a.qterm[qrow] : make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int e = 0; e < KL; ++e) {
       sc.la[u][e] = INFINITY;
@@ -460,6 +523,7 @@ __global__ __launch_bounds__(64 * screen_waves<KS>()) void k_screen(SArgs a) {
     sc.thri[u] = max(__float_as_int(sc.thr[u]), 0);
     sc.cnt[u] = 0;
   }
+  sc.kk = a.qterm && a.k <= (uint32_t)KL ? a.k : 0u;
   sc.buf = s_buf + tid;
 
   // Local tile i of slice y is global point tile tb + i ts.  The rows are in
@@ -991,7 +1055,7 @@ size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp) {
   const uint64_t ks = (uint64_t)screen_ks(t.d);
   const uint32_t n_tiles = (uint32_t)((t.n + 31) / 32);
   return 2 * scratch_piece(nq * 2 * slices * KP * 4) + scratch_piece(nq * 4) + scratch_piece(nq * CAND_CAP * 4) +
-         scratch_piece(nq) + scratch_piece(4) + scratch_piece(nq * 2 * ks * 2) + scratch_piece(nq * 8) +
+         scratch_piece(nq) + scratch_piece(4) + scratch_piece(nq * 2 * ks * 2) + 2 * scratch_piece(nq * 8) +
          (screen_seeded(n_tiles) ? scratch_piece(nq * 4) : 0);
 }
 
@@ -1020,8 +1084,10 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   const int ks = screen_ks(t.d);
   uint16_t* qrows = nullptr;
   uint2* qnorm = nullptr;
+  float2* qterm = nullptr;
   if (e == hipSuccess) e = sc.get(&qrows, nq * 2 * ks);
   if (e == hipSuccess) e = sc.get(&qnorm, nq);
+  if (e == hipSuccess) e = sc.get(&qterm, nq);
   const bool seeded = screen_seeded(n_tiles);
   float* thr0 = nullptr;
   if (e == hipSuccess && seeded) e = sc.get(&thr0, nq);
@@ -1031,15 +1097,18 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   }
   {
     const uint32_t pb = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 4096);
-    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, (uint32_t)t.d, t.frame, qrows, qnorm);
-    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, (uint32_t)t.d, t.frame, qrows, qnorm);
-    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, (uint32_t)t.d, t.frame, qrows, qnorm);
+    const uint32_t D = (uint32_t)t.d;
+    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, t.scr_pc, qrows, qnorm, qterm);
+    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, t.scr_pc, qrows, qnorm, qterm);
+    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, t.scr_pc, qrows, qnorm, qterm);
   }
   SArgs sa{};
   sa.rows_h = t.rows_h;
   sa.pnorm_h = t.pnorm_h;
   sa.qrows = reinterpret_cast<const uint4*>(qrows);
   sa.qnorm = qnorm;
+  sa.qterm = qterm;
+  sa.k = k;
   sa.part_a = part_a;
   sa.part_i = part_i;
   sa.nq = nq;

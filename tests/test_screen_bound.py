@@ -316,3 +316,86 @@ def test_integer_group_minimum_needs_the_clamp():
     A, thr = A.ravel(), thr.ravel()
     under = A < thr
     assert not np.all(_bits(A)[under] < _bits(thr)[under])
+
+
+# The main pass lowers a lane's threshold at each merge to a bound of tau_A
+# from its list's k-th entry ak (an upper bound of A_k), computed in FP32
+# (screen_tau_a32 in kdt_screen.hip) from the query's e1 and E2 rounded up to
+# float.  Values at or above it are dropped, so it must lie strictly above the
+# tau_A that the select computes in FP64 from A_k <= ak (cutoff +inf, which
+# can only lower it).  Emulated with numpy's float32 (round to nearest even),
+# and once with every FP32 result one ulp lower still (sqrtf is only
+# specified within one ulp, and the compiler may contract the products).
+
+_F32 = np.float32
+
+
+def _round_up_f32(x: float) -> np.float32:
+    f = _F32(x)
+    if float(f) < x:
+        f = np.nextafter(f, _F32(np.inf))
+    return f
+
+
+def _select_tau_a(ak: float, e1: float, E2: float, D: int, sigma: float) -> np.float32:
+    """k_screen_select's tau_A (FP64, then rounded up to float) for A_k = ak, no cutoff."""
+    sh = 1.0 + 2.0 * (D + 3.0) * 2.0 ** -53
+    if not ak < math.inf:
+        return _F32(np.inf)
+    r = (math.sqrt(max(0.0, ak + E2)) + e1) / sigma
+    tau = sh * r * r * (1 + 1e-12)
+    if not tau < math.inf:
+        return _F32(np.inf)
+    r2 = sigma * math.sqrt(sh * tau) + e1
+    t = (r2 * r2 + E2) * (1 + 1e-12)
+    return _round_up_f32(t) if t < math.inf else _F32(np.inf)
+
+
+def _merge_tau_a32(ak, e1f, E2f, low=False, margin=True, two_e1=True):
+    """screen_tau_a32: each FP32 operation rounded to nearest, or (low) one ulp lower."""
+    def op(x):
+        x = _F32(x)
+        return np.nextafter(x, _F32(-np.inf)) if low and np.isfinite(x) else x
+    with np.errstate(over="ignore", invalid="ignore"):
+        r = op(op(np.sqrt(max(op(_F32(ak) + E2f), _F32(0.0)))) + e1f)
+        w = op(r + e1f) if two_e1 else r
+        v = op(op(w * w) + E2f)
+        return op(v * _F32(1.0 + 2.0 ** -18)) if margin else v
+
+
+def _merge_cases():
+    rng = np.random.default_rng(91)
+    for D in (8, 12, 16, 31, 32, 48, 64):
+        for _ in range(40):
+            sigma = 2.0 ** int(rng.integers(-30, 31))
+            spread = 2.0 ** rng.uniform(-12, 11) * math.sqrt(D)  # sigma (|q - c| + P_c)
+            qc = spread * rng.uniform(0.0, 1.0) / sigma
+            pc = spread / sigma - qc
+            E2, e1, _ = bounds(D, qc, pc, sigma)
+            aks = [-E2 * 2.0, -E2, -E2 * rng.uniform(), 0.0, E2 * rng.uniform(), e1 * e1,
+                   spread * spread * 2.0 ** rng.uniform(-40, 1), spread * spread * 4.0, 1e30, 3e38, math.inf]
+            aks += list(spread * spread * 2.0 ** rng.uniform(-30, 2, size=8))
+            for ak in aks:
+                yield D, sigma, e1, E2, float(_F32(ak))
+
+
+def _merge_bound_holds(**variant) -> bool:
+    for D, sigma, e1, E2, ak in _merge_cases():
+        ta = _select_tau_a(ak, e1, E2, D, sigma)
+        e1f, E2f = _round_up_f32(e1), _round_up_f32(E2)
+        for low in (False, True):
+            lim = _merge_tau_a32(ak, e1f, E2f, low=low, **variant)
+            if not (lim > ta or (np.isinf(ta) and np.isinf(lim))):
+                return False
+    return True
+
+
+def test_merge_bound_exceeds_select_threshold():
+    assert _merge_bound_holds()
+
+
+@pytest.mark.parametrize("weakened", ["margin", "two_e1"])
+def test_weakened_merge_bound_fails(weakened):
+    """Without the 1 + 2^-18 factor, or with one e1 instead of two, the FP32
+    bound falls to or below the select's tau_A on these inputs."""
+    assert not _merge_bound_holds(**{weakened: False})

@@ -272,30 +272,43 @@ __device__ __forceinline__ float round_up_f(double x) {
   return f;
 }
 
-// The screen's error terms for one query (the bound in the file header),
-// from qc2 = the sum of fl64(q_j - c_j)^2 in any order (|q - c| is rounded up
-// by a relative 2^-44, as P_c in the build): e1 >= | |q^ - p^| - sigma |q - p| |
-// and E2 >= |A - |q^ - p^|^2| for every row p.  Each is rounded up by a
-// relative 1e-12 (far above the FP64 error of computing it), then to float.
-__device__ __forceinline__ float2 screen_terms(double qc2, uint32_t D_, double sg, double pc) {
+// The per-tree factors of the screen's error terms (the bound in the file
+// header), formed on the host: c1 = a sigma, eb = 2 b sqrt(D) and
+// g = (1 + 2^-7) gamma + 1.13 2^-24, in the operation order of k_screen_select.
+struct TermK {
+  double c1, eb, g, sg, pc;
+};
+
+TermK screen_term_k(uint32_t D_, double sg, double pc) {
   const double D = (double)D_;
-  const double qc = sqrt(qc2) * (1.0 + 0x1p-44);
   const double ea = 0x1p-11 * (1.0 + 0x1p-12);
-  const double eb = 0x1p-14 * (1.0 + 0x1p-60) * 2.0 * sqrt(D) * (1.0 + 0x1p-50);
-  const double e1 = (ea * sg * (qc + pc) + eb) * (1.0 + 1e-12);
-  const double Nh = (sg * (qc + pc) * (1.0 + ea) + eb) * (1.0 + 1e-12);
   const double m = (D + 5.0) * 0x1p-23;
   const double gam = m / (1.0 - m);
-  const double E2 = (((1.0 + 0x1p-7) * gam + 1.13 * 0x1p-24) * Nh * Nh + 0x1p-118) * (1.0 + 1e-12);
+  return TermK{ea * sg, 0x1p-14 * (1.0 + 0x1p-60) * 2.0 * sqrt(D) * (1.0 + 0x1p-50),
+               (1.0 + 0x1p-7) * gam + 1.13 * 0x1p-24, sg, pc};
+}
+
+// One query's (e1, E2) from qc2 = the sum of fl64(q_j - c_j)^2 in any order
+// (|q - c| rounded up by a relative 2^-44, as P_c in the build), each rounded
+// up to float: e1 >= | |q^ - p^| - sigma |q - p| | and E2 >= |A - |q^ - p^|^2|
+// for every row p.  k_screen_select forms the same bounds from its own sum of
+// the squares, so the two agree to a relative 2^-40.
+__device__ __forceinline__ float2 screen_terms(double qc2, const TermK& t) {
+  const double ea = 0x1p-11 * (1.0 + 0x1p-12);
+  const double qc = sqrt(qc2) * (1.0 + 0x1p-44);
+  const double e1 = (t.c1 * (qc + t.pc) + t.eb) * (1.0 + 1e-12);
+  const double Nh = (t.sg * (qc + t.pc) * (1.0 + ea) + t.eb) * (1.0 + 1e-12);
+  const double E2 = (t.g * Nh * Nh + 0x1p-118) * (1.0 + 1e-12);
   return make_float2(round_up_f(e1), round_up_f(E2));
 }
 
 // A float strictly above the tau_A that k_screen_select derives from an
 // upper bound ak of A_k (with no cutoff, which only lowers it; e1, E2 as
-// screen_terms gives them, >= the select's): that value is
+// screen_terms gives them): with the select's own terms that value is
 //   T = ((s' (sqrt(ak + E2) + e1) + e1)^2 + E2) (1 + 1e-12),  s' <= 1 + 2^-44,
-// rounded up to float (one ulp, <= 2^-23 relative), so T <= (w^2 + E2)
-// (1 + 2^-22) with w = sqrt(max(ak + E2, 0)) + 2 e1.  Here the seven FP32
+// rounded up to float (one ulp, <= 2^-23 relative), so with these terms
+// (2^-40 of the select's) T <= (w^2 + E2)(1 + 2^-22) with
+// w = sqrt(max(ak + E2, 0)) + 2 e1.  Here the seven FP32
 // operations each err by at most 2^-23 relatively (sqrtf within one ulp, the
 // rest within half an ulp; E2 >= 2^-118 keeps the radicand's rounding, when
 // it is subnormal, far below e1's share of w), so the result is >= (w^2 +
@@ -314,11 +327,15 @@ __device__ __forceinline__ float screen_tau_a32(float ak, float2 terms) {
 // query's error terms (e1, E2) of screen_terms, for the main pass's bounds.
 template <int KS>
 __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint32_t D,
-                                 const double* __restrict__ frame, double pc, uint16_t* __restrict__ qrows,
+                                 const double* __restrict__ frame, TermK tk, uint16_t* __restrict__ qrows,
                                  uint2* __restrict__ qnorm, float2* __restrict__ qterm) {
   const double sigma = frame[D];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) {
     double qn = 0.0, qc2 = 0.0;
+    // (Unrolled by 4 at most: a fully unrolled 16-column loop holds every load
+    // in flight and takes 68 registers, too many to share a compute unit with
+    // a running screen pass.)
+#pragma unroll 4
     for (uint32_t c = 0; c < 2 * KS; ++c) {
       const double y = c < D ? q[i * D + c] - frame[c] : 0.0;
       const _Float16 v = c < D ? screen_f16(y, sigma) : (_Float16)0.0f;
@@ -328,7 +345,7 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
     }
     const uint2 w = screen_norm_words((float)qn);
     qnorm[i] = make_uint2((w.x >> 16) | (w.y << 16), (w.y >> 16) | 0x3F800000u);
-    qterm[i] = screen_terms(qc2, D, sigma, pc);
+    qterm[i] = screen_terms(qc2, tk);
   }
 }
 
@@ -1098,9 +1115,10 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   {
     const uint32_t pb = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 4096);
     const uint32_t D = (uint32_t)t.d;
-    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, t.scr_pc, qrows, qnorm, qterm);
-    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, t.scr_pc, qrows, qnorm, qterm);
-    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, t.scr_pc, qrows, qnorm, qterm);
+    const TermK tk = screen_term_k(D, t.scr_sigma, t.scr_pc);
+    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm);
+    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm);
+    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm);
   }
   SArgs sa{};
   sa.rows_h = t.rows_h;

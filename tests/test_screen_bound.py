@@ -321,7 +321,8 @@ def test_integer_group_minimum_needs_the_clamp():
 # The main pass lowers a lane's threshold at each merge to a bound of tau_A
 # from its list's k-th entry ak (an upper bound of A_k), computed in FP32
 # (screen_tau_a32 in kdt_screen.hip) from the query's e1 and E2 rounded up to
-# float.  Values at or above it are dropped, so it must lie strictly above the
+# float (formed in FP64 from a sum of squares in another order than the
+# select's: within a relative 2^-40 of the select's terms).  Values at or above it are dropped, so it must lie strictly above the
 # tau_A that the select computes in FP64 from A_k <= ak (cutoff +inf, which
 # can only lower it).  Emulated with numpy's float32 (round to nearest even),
 # and once with every FP32 result one ulp lower still (sqrtf is only
@@ -381,7 +382,9 @@ def _merge_cases():
 
 def _merge_bound_holds(**variant) -> bool:
     for D, sigma, e1, E2, ak in _merge_cases():
-        ta = _select_tau_a(ak, e1, E2, D, sigma)
+        # the select forms e1 and E2 from its own sum of the squares of q - c:
+        # up to a relative 2^-40 above the main pass's
+        ta = _select_tau_a(ak, e1 * (1 + 2.0 ** -40), E2 * (1 + 2.0 ** -40), D, sigma)
         e1f, E2f = _round_up_f32(e1), _round_up_f32(E2)
         for low in (False, True):
             lim = _merge_tau_a32(ak, e1f, E2f, low=low, **variant)

@@ -136,12 +136,14 @@ constexpr int screen_waves() { return KS <= 16 ? 4 : KDT_SCREEN_SW32; }
 // slots, below).  Rows of 16
 // columns: 4 (2 matrix instructions per tile; 2^21 16-d normal rows, k = 10:
 // 65,536 queries 13.3 ms vs 14.7 with 1 and 13.9 with 2, 8,192 queries 2.27
-// vs 2.60 and 2.39 ms).  Rows of 32 columns: 1.
+// vs 2.60 and 2.39 ms; 2 and 8 again with the rows loaded straight into LDS:
+// 1-10% slower or mixed).  Rows of 32 columns: 2 (2^20 32-d normal rows,
+// 32,768 queries, k = 10: 4.09 ms vs 4.38-4.57 with 1 and 4.14 with 4).
 #ifndef KDT_SCREEN_TS8
 #define KDT_SCREEN_TS8 4
 #endif
 #ifndef KDT_SCREEN_TS16
-#define KDT_SCREEN_TS16 1
+#define KDT_SCREEN_TS16 2
 #endif
 #ifndef KDT_SCREEN_TS32
 #define KDT_SCREEN_TS32 4

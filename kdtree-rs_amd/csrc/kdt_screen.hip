@@ -325,12 +325,14 @@ __device__ __forceinline__ float screen_tau_a32(float ak, float2 terms) {
 // slice reads them): row i as -2 q^ (q^_j = screen_f16(fl64(q_j - c_j),
 // sigma), exact in fp16) over 2 KS columns (zero past D), and the norm block's
 // query-side words of qn = fl32(|q^|^2) (summed in FP64): (q1 | q2 << 16,
-// q3 | 1 << 16), from the three bf16 pieces of screen_norm_words; and the
-// query's error terms (e1, E2) of screen_terms, for the main pass's bounds.
+// q3 | 1 << 16), from the three bf16 pieces of screen_norm_words; the
+// query's error terms (e1, E2) of screen_terms, for the main pass's bounds;
+// and, when thr0 is given, +inf as the query's shared threshold for a
+// seeding pass.
 template <int KS>
 __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint32_t D,
                                  const double* __restrict__ frame, TermK tk, uint16_t* __restrict__ qrows,
-                                 uint2* __restrict__ qnorm, float2* __restrict__ qterm) {
+                                 uint2* __restrict__ qnorm, float2* __restrict__ qterm, float* __restrict__ thr0) {
   const double sigma = frame[D];
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += (uint64_t)gridDim.x * blockDim.x) {
     double qn = 0.0, qc2 = 0.0;
@@ -348,6 +350,7 @@ __global__ void k_screen_queries(const double* __restrict__ q, uint64_t nq, uint
     const uint2 w = screen_norm_words((float)qn);
     qnorm[i] = make_uint2((w.x >> 16) | (w.y << 16), (w.y >> 16) | 0x3F800000u);
     qterm[i] = screen_terms(qc2, tk);
+    if (thr0) thr0[i] = INFINITY;
   }
 }
 
@@ -1114,13 +1117,24 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     msg = std::string("screen scratch: ") + hipGetErrorString(e);
     return 100;
   }
+  // Seeding pass geometry (below): S0 slices over every SEED_STRIDE-th tile.
+  const uint32_t ns = (n_tiles + SEED_STRIDE - 1) / SEED_STRIDE;
+  const uint32_t s0 = (uint32_t)std::min<uint64_t>(slices, std::max<uint32_t>(1, ns / 8));
+  const uint32_t seed_kl = seed_list_len(s0, k);
+  // The seeding lists share a threshold per query (the seed's slot, +inf
+  // until then) when they hold k entries or more: a value at or above some
+  // list's k-th entry, or above a bound of tau_A from it, is not among the k
+  // smallest entries of all the lists, so the seed is the same.  Shorter
+  // lists keep their own thresholds (sharing their last entry could drop
+  // some of those k smallest and weaken the seed).
+  const bool seed_shared = seeded && seed_kl >= k;
   {
     const uint32_t pb = (uint32_t)std::min<uint64_t>((nq + 255) / 256, 4096);
     const uint32_t D = (uint32_t)t.d;
     const TermK tk = screen_term_k(D, t.scr_sigma, t.scr_pc);
-    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm);
-    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm);
-    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm);
+    if (ks == 8) k_screen_queries<8><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm, seed_shared ? thr0 : nullptr);
+    else if (ks == 16) k_screen_queries<16><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm, seed_shared ? thr0 : nullptr);
+    else k_screen_queries<32><<<pb, 256, 0, s>>>(trav.q, nq, D, t.frame, tk, qrows, qnorm, qterm, seed_shared ? thr0 : nullptr);
   }
   SArgs sa{};
   sa.rows_h = t.rows_h;
@@ -1184,15 +1198,13 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     // Seeding pass: every SEED_STRIDE-th point tile, in S0 slices, then each
     // query's tau_A of the k-th A over those lists as every lane's starting
     // threshold in the full pass (the lists' scratch is reused).
-    const uint32_t ns = (n_tiles + SEED_STRIDE - 1) / SEED_STRIDE;
-    const uint32_t s0 = (uint32_t)std::min<uint64_t>(slices, std::max<uint32_t>(1, ns / 8));
     SArgs sp = sa;
-    sp.thr0 = nullptr;
+    sp.thr0 = seed_shared ? thr0 : nullptr;
     sp.slices = s0;
     sp.tb_mul = SEED_STRIDE;
     sp.ts = SEED_STRIDE * s0;
     sp.tmax = 0xFFFFFFFFu;
-    const uint32_t kl = seed_list_len(s0, k);
+    const uint32_t kl = seed_kl;
     screen(sp, s0, kl, "k_screen_seed", 2.0 * (double)t.d * 32.0 * (double)ns * (double)nq);
     SelArgs ss = se;
     ss.lists = 2 * s0;

@@ -638,20 +638,24 @@ struct SelArgs {
   double sigma, pc, cutoff;  // sigma, P_c >= max over rows of |p - c|, the reduced cutoff
 };
 
-// One wave per query; lane l reads lists l, l + 64, ... (at most LISTS_PER_LANE):
+// One wave per query; lane l reads lists l, l + 64, ... (at most LPL <=
+// LISTS_PER_LANE; the launch takes the smallest LPL of 1, 4, 16 that covers the
+// lists, so that a call with few lists neither loops over nor holds registers
+// for absent ones):
 // k rounds of wave-min over the list heads give A_k, then tau_A from the error
 // bound, the completeness check and the candidates (list entries with
 // A <= tau_A, at most CAND_CAP).
+template <int LPL>
 __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   const uint64_t qi = blockIdx.x;
   const uint32_t lane = threadIdx.x;
   const uint32_t kl = a.kl;
   const float* la0 = a.part_a + qi * a.lists * kl;
   const uint32_t* li0 = a.part_i + qi * a.lists * kl;
-  uint32_t head[LISTS_PER_LANE];
-  float hv[LISTS_PER_LANE];  // value at each head (+inf when exhausted or absent)
+  uint32_t head[LPL];
+  float hv[LPL];  // value at each head (+inf when exhausted or absent)
 #pragma unroll
-  for (int c = 0; c < LISTS_PER_LANE; ++c) {
+  for (int c = 0; c < LPL; ++c) {
     const uint32_t L = lane + 64 * c;
     head[c] = 0;
     hv[c] = L < a.lists ? la0[L * kl] : INFINITY;
@@ -660,7 +664,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   for (uint32_t r = 0; r < a.k; ++r) {
     float cur = hv[0];
 #pragma unroll
-    for (int c = 1; c < LISTS_PER_LANE; ++c) cur = fminf(cur, hv[c]);
+    for (int c = 1; c < LPL; ++c) cur = fminf(cur, hv[c]);
     float m = cur;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o));
@@ -670,7 +674,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
     if (lane == (uint32_t)(__ffsll((unsigned long long)hit) - 1)) {
       bool done = false;
 #pragma unroll
-      for (int c = 0; c < LISTS_PER_LANE; ++c) {
+      for (int c = 0; c < LPL; ++c) {
         if (!done && hv[c] == m) {
           done = true;
           ++head[c];
@@ -726,11 +730,11 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   }
   // Completeness: a full list whose last value is <= tau_A may have dropped a
   // candidate.  Unfilled slots (A = +inf, position 0xFFFFFFFF) end a list.
-  uint32_t cnt_c[LISTS_PER_LANE];
+  uint32_t cnt_c[LPL];
   uint32_t mcount = 0;
   bool complete = true;
 #pragma unroll
-  for (int c = 0; c < LISTS_PER_LANE; ++c) {
+  for (int c = 0; c < LPL; ++c) {
     const uint32_t L = lane + 64 * c;
     uint32_t n = 0;
     if (L < a.lists) {
@@ -756,7 +760,7 @@ __global__ __launch_bounds__(64) void k_screen_select(SelArgs a) {
   if (good) {
     uint32_t off = incl - mcount;
 #pragma unroll
-    for (int c = 0; c < LISTS_PER_LANE; ++c) {
+    for (int c = 0; c < LPL; ++c) {
       const uint32_t* li = li0 + (lane + 64 * c) * kl;
       for (uint32_t e = 0; e < cnt_c[c]; ++e) a.cand[qi * CAND_CAP + off + e] = li[e];
       off += cnt_c[c];
@@ -1081,6 +1085,12 @@ size_t screen_scratch_bytes(const Tree& t, const QueryParams& qp) {
          (screen_seeded(n_tiles) ? scratch_piece(nq * 4) : 0);
 }
 
+void launch_select(const SelArgs& a, hipStream_t s) {
+  if (a.lists <= 64) k_screen_select<1><<<(uint32_t)a.nq, 64, 0, s>>>(a);
+  else if (a.lists <= 256) k_screen_select<4><<<(uint32_t)a.nq, 64, 0, s>>>(a);
+  else k_screen_select<LISTS_PER_LANE><<<(uint32_t)a.nq, 64, 0, s>>>(a);
+}
+
 int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scratch& sc, hipStream_t s,
                   std::string& msg) {
   const uint64_t nq = qp.nq;
@@ -1210,7 +1220,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
     ss.lists = 2 * s0;
     ss.kl = kl;
     ss.seed = thr0;
-    k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(ss);
+    launch_select(ss, s);
     sa.thr0 = thr0;
   }
   // Algorithmic work: 2 D FLOPs per (query, point) pair of the contraction.
@@ -1220,7 +1230,7 @@ int launch_screen(const Tree& t, const QueryParams& qp, const KArgs& trav, Scrat
   launch_zero_u32(nflag, 1, s);
   {
     KernelTimer tm(s);
-    k_screen_select<<<(uint32_t)nq, 64, 0, s>>>(se);
+    launch_select(se, s);
     tm.stop("k_screen_select", (double)nq);
   }
 
